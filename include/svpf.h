@@ -1,0 +1,95 @@
+/* libsvpf — MI355X (gfx950) implementation of the open-svpflow motion path:
+ * svpflow1.Super pyramid build, svpflow1.Analyse block search and
+ * svpflow2.SmoothFps motion-compensated render.
+ *
+ * Plain C ABI: no exceptions, int status codes (0 = ok, negative = error, text
+ * in svpf_last_error()), caller-owned buffers with explicit byte strides, and
+ * opaque context handles.  Every entry point is safe to call from several host
+ * threads as long as each thread uses its own context (a context owns one HIP
+ * stream); calls on one context are serialised by the caller.
+ *
+ * Pointers named d_* are device pointers (allocated with svpf_device_alloc or any
+ * HIP allocator on the context's device); h_* are host pointers.
+ *
+ * Each entry point cites the reference function it replaces (paths relative to
+ * the open-svpflow repository).
+ */
+#ifndef SVPF_H
+#define SVPF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SVPF_OK 0
+#define SVPF_ERR_ARG (-1)    /* invalid argument / unsupported shape */
+#define SVPF_ERR_HIP (-2)    /* HIP runtime error (message names the call) */
+#define SVPF_ERR_OPT (-3)    /* option string rejected (message = reference error text) */
+#define SVPF_ERR_NOMEM (-4)  /* allocation failed */
+#define SVPF_ERR_STATE (-5)  /* call out of order / resource exhausted */
+
+typedef struct svpf_ctx svpf_ctx;
+
+/* One YUV420P8 frame: three planes with byte strides. */
+typedef struct svpf_frame {
+    uint8_t* y;
+    uint8_t* u;
+    uint8_t* v;
+    int32_t stride_y;
+    int32_t stride_u;
+    int32_t stride_v;
+} svpf_frame;
+
+/* ------------------------------------------------------------------ runtime */
+const char* svpf_last_error(void);
+int svpf_version(void); /* svpGetVersion value, crates/svpflow1/src/abi.rs:3-8 */
+int svpf_create(int device, svpf_ctx** out);
+void svpf_destroy(svpf_ctx* ctx);
+int svpf_synchronize(svpf_ctx* ctx);
+void* svpf_stream(svpf_ctx* ctx); /* the context's hipStream_t */
+int svpf_device_alloc(svpf_ctx* ctx, size_t bytes, void** d_out);
+int svpf_device_free(svpf_ctx* ctx, void* d_ptr);
+int svpf_copy_h2d(svpf_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
+int svpf_copy_d2h(svpf_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
+int svpf_memset_d(svpf_ctx* ctx, void* d_dst, int value, size_t bytes);
+/* Event timing on the context's stream: start, then stop returns elapsed ms. */
+int svpf_timer_start(svpf_ctx* ctx);
+int svpf_timer_stop(svpf_ctx* ctx, float* ms);
+
+/* ------------------------------------------------------------------ Super */
+/* Replaces SuperOpts (crates/svpflow1/src/super_opts.rs:3-95). */
+typedef struct svpf_super_params {
+    int32_t width, height; /* source luma size */
+    int32_t pel;           /* 1, 2 or 4 */
+    int32_t gpu;           /* 0..2 */
+    int32_t scale_down;    /* 4 = build the pyramid */
+    int32_t scale_up;      /* 0 bilinear, 2 bicubic (forced 0 when gpu != 0) */
+    int32_t full;          /* bool */
+    int32_t rc;            /* bool */
+    int32_t levels;        /* auto_levels(width, height) */
+} svpf_super_params;
+
+/* SuperOpts::from_opt (super_opts.rs:17-49) over the reference's option-string
+ * syntax (crates/svpflow-core/src/params.rs:1-11).  On SVPF_ERR_OPT `err` holds
+ * the reference's message.  *sdata = SuperOpts::pack_data (super_opts.rs:51-63). */
+int svpf_super_params_from_opt(const char* opt, size_t opt_len, int32_t width, int32_t height,
+                               svpf_super_params* out, int64_t* sdata, char* err, size_t err_len);
+/* super_plane_height (super_opts.rs:89-95): rows of the luma super plane; chroma has half. */
+int32_t svpf_super_height(const svpf_super_params* p);
+
+/* build_plane x3 for one frame (super_build.rs:8-93 as called by get_frame_super,
+ * super_filter.rs:143-268).  d_src/d_dst hold device plane pointers.  Every byte of
+ * the super frame (stride x rows) is written, so the destination needs no clearing. */
+int svpf_super_build(svpf_ctx* ctx, const svpf_super_params* p, const svpf_frame* d_src,
+                     const svpf_frame* d_dst);
+/* The same for n frames of one clip in one launch per pyramid stage (batched form). */
+int svpf_super_build_batch(svpf_ctx* ctx, const svpf_super_params* p, int32_t n, const svpf_frame* d_src,
+                           const svpf_frame* d_dst);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SVPF_H */

@@ -1,0 +1,250 @@
+// Super pyramid build for gfx950 (replaces crates/svpflow1/src/super_build.rs:8-362).
+//
+// Two kernels, both writing every byte of their rows (so the super frame needs no
+// clearing pass; the reference zeroes the plane first, super_filter.rs:210-223):
+//   super_level0_kernel  one workgroup per (source row y, plane, frame): stages the
+//                        six full-pel rows y-2..y+3 (edge-padded to the stride) in LDS
+//                        and writes row y of every level-0 subplane.
+//   super_reduce_kernel  one workgroup per (destination row, plane, frame) of level
+//                        l+1: vertical 6-tap into LDS, then the horizontal 6-tap with
+//                        the reference's first-tap rule, written over the full stride.
+// Rows are written as 32-bit words when the row start is 4-byte aligned, bytes otherwise.
+#include "svpf_internal.h"
+
+namespace svpf {
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ uint8_t avg8(uint32_t a, uint32_t b) { return (uint8_t)((a + b + 1) >> 1); }
+
+struct L0Args {
+    SuperGeom g;
+    const Planes* src;
+    const Planes* dst;
+};
+
+// Full-pel region F(r, c) = src[r][min(c, w-1)] for c < stride (super_build.rs:45-67:
+// copy then pad with the edge byte).  Rows y-2..y+3 come from LDS, others (only the
+// first rows, read where the pel fill runs past the last row) from global memory.
+struct Rows {
+    const uint8_t* lds;  // 6 rows of `pitch` bytes
+    const uint8_t* sp;   // source plane
+    int ss, pitch, y, h, w, ds;
+    __device__ __forceinline__ uint8_t F(int r, int c) const {
+        int k = r - y + 2;
+        if (k >= 0 && k < 6) return lds[k * pitch + c];
+        return sp[(size_t)r * ss + min(c, w - 1)];
+    }
+    // x-half subplane byte (super_build.rs:183-192); beyond w the row keeps its zero.
+    // The read of x+1 follows the buffer: past the stride it is the next row's first
+    // byte, and past the last row it is the first x-half byte.
+    __device__ __forceinline__ uint8_t hp(int yy, int x) const {
+        if (x >= w) return 0;
+        uint8_t b;
+        if (x + 1 < ds) b = F(yy, x + 1);
+        else if (yy + 1 < h) b = F(yy + 1, 0);
+        else b = avg8(F(0, 0), ds > 1 ? F(0, 1) : F(1, 0));
+        return avg8(F(yy, x), b);
+    }
+};
+
+__global__ void __launch_bounds__(kThreads) super_level0_kernel(L0Args a) {
+    const int y = blockIdx.x;
+    const int plane = blockIdx.y;
+    const int frame = blockIdx.z;
+    const SuperGeom& g = a.g;
+    const int w = g.level_w[plane][0], h = g.level_h[plane][0];
+    if (y >= h) return;
+    const Planes src = a.src[frame];
+    const Planes dst = a.dst[frame];
+    const int ds = dst.stride[plane];
+    const int ss = src.stride[plane];
+    const uint8_t* sp = src.p[plane];
+    uint8_t* dp = dst.p[plane];
+
+    extern __shared__ uint8_t lds[];
+    const int pitch = ds + 4;
+    for (int k = 0; k < 6; k++) {  // rows y-2..y+3 (clamped), edge-padded to the stride
+        int r = min(max(y - 2 + k, 0), h - 1);
+        const uint8_t* s = sp + (size_t)r * ss;
+        for (int c = threadIdx.x; c < pitch; c += kThreads) lds[k * pitch + c] = s[min(c, w - 1)];
+    }
+    __syncthreads();
+    const Rows R{lds, sp, ss, pitch, y, h, w, ds};
+
+    const int pel = g.pel;
+    const int nsub = g.full ? pel * pel : 1;
+    const bool fill = pel >= 2 && g.gpu <= 1 && g.full;
+    const bool bilinear = fill && g.scale_up == 0;
+    const bool bicubic = fill && g.scale_up == 2;
+
+    for (int sub = 0; sub < nsub; sub++) {
+        uint8_t* row = dp + (size_t)(g.level_row[plane][0] + sub * h + y) * ds;
+        auto value = [&](int x) -> uint8_t {
+            if (sub == 0) return R.F(y, x);
+            if (sub > 3 || x >= w) return 0;
+            if (bilinear) {
+                if (sub == 1) return R.hp(y, x);
+                if (sub == 2) {  // super_build.rs:194-209: row 0 is then replaced by the full row 0
+                    if (y == 0) return R.F(0, x);
+                    return avg8(R.F(y, x), (y + 1 < h) ? R.F(y + 1, x) : R.hp(0, x));
+                }
+                // sub == 3, super_build.rs:211-223: row 0 is the full row 0 shifted by one byte
+                if (y == 0) return (x + 1 < ds) ? R.F(0, x + 1) : (h > 1 ? R.F(1, 0) : R.hp(0, 0));
+                uint8_t b;
+                if (x + 1 < ds) {
+                    b = (y + 1 < h) ? R.F(y + 1, x + 1) : R.hp(0, x + 1);
+                } else {  // stride == w: the diagonal read wraps to row y+2, column 0
+                    b = (y + 2 < h) ? R.F(y + 2, 0) : R.hp(y + 2 - h, 0);
+                }
+                return avg8(R.F(y, x), b);
+            }
+            if (bicubic) {  // super_build.rs:95-177 (`half` per pixel)
+                auto vert = [&](int xx) -> uint8_t {
+                    if (y < 2 || y + 4 >= h) return (y + 1 == h) ? R.F(y, xx) : avg8(R.F(y, xx), R.F(y + 1, xx));
+                    int v = (int)R.F(y - 2, xx) + (int)R.F(y + 3, xx) +
+                            5 * (4 * (int)R.F(y, xx) - (int)R.F(y - 1, xx) + 4 * (int)R.F(y + 1, xx) -
+                                 (int)R.F(y + 2, xx)) + 16;
+                    return (uint8_t)min(255, max(0, v >> 5));
+                };
+                auto horiz = [&](auto get, int xx) -> uint8_t {
+                    if (xx < 2 || xx + 4 >= w) return (xx + 1 == w) ? get(xx) : avg8(get(xx), get(xx + 1));
+                    int v = (int)get(xx - 2) + (int)get(xx + 3) +
+                            5 * (4 * (int)get(xx) - (int)get(xx - 1) + 4 * (int)get(xx + 1) - (int)get(xx + 2)) + 16;
+                    return (uint8_t)min(255, max(0, v >> 5));
+                };
+                if (sub == 1) return horiz([&](int xx) { return R.F(y, xx); }, x);
+                if (sub == 2) return vert(x);
+                return horiz(vert, x);
+            }
+            return 0;
+        };
+        if ((((uintptr_t)row) & 3) == 0) {
+            for (int c = threadIdx.x * 4; c < ds; c += kThreads * 4) {
+                if (c + 4 <= ds) {
+                    uint32_t wv = (uint32_t)value(c) | ((uint32_t)value(c + 1) << 8) | ((uint32_t)value(c + 2) << 16) |
+                                  ((uint32_t)value(c + 3) << 24);
+                    *reinterpret_cast<uint32_t*>(row + c) = wv;
+                } else {
+                    for (int k = c; k < ds; k++) row[k] = value(k);
+                }
+            }
+        } else {
+            for (int c = threadIdx.x; c < ds; c += kThreads) row[c] = value(c);
+        }
+    }
+}
+
+struct ReduceArgs {
+    SuperGeom g;
+    const Planes* dst;
+    int level;  // source level; writes level+1
+};
+
+// reduce_6tap + horizontal_6tap_inplace (super_build.rs:254-357) for one destination row.
+__global__ void __launch_bounds__(kThreads) super_reduce_kernel(ReduceArgs a) {
+    const int dy = blockIdx.x;
+    const int plane = blockIdx.y;
+    const int frame = blockIdx.z;
+    const SuperGeom& g = a.g;
+    const int l = a.level;
+    const int src_w = g.level_w[plane][l], src_h = g.level_h[plane][l];
+    const int dst_w = g.level_w[plane][l + 1], dst_h = g.level_h[plane][l + 1];
+    if (dy >= dst_h) return;
+    const Planes P = a.dst[frame];
+    const int ds = P.stride[plane];
+    uint8_t* base = P.p[plane];
+    const uint8_t* srow0 = base + (size_t)g.level_row[plane][l] * ds;
+    uint8_t* drow = base + (size_t)(g.level_row[plane][l + 1] + dy) * ds;
+
+    extern __shared__ uint8_t inter[];  // src_w bytes: the vertically filtered row
+    const int sy = dy * 2;
+    const bool use_edge = dy == 0 || dy + 1 == dst_h || sy - 2 < 0 || sy + 3 >= src_h;
+    for (int x = threadIdx.x; x < src_w; x += kThreads) {
+        uint8_t v;
+        if (use_edge) {
+            int y0 = min(sy, src_h - 1), y1 = min(sy + 1, src_h - 1);
+            v = avg8(srow0[(size_t)y0 * ds + x], srow0[(size_t)y1 * ds + x]);
+        } else {
+            const uint8_t* c = srow0 + (size_t)(sy - 2) * ds + x;
+            uint32_t p0 = c[0], p1 = c[ds], p2 = c[2 * ds], p3 = c[3 * ds], p4 = c[4 * ds], p5 = c[5 * ds];
+            v = (uint8_t)min(255u, (p0 + p5 + 5 * (p1 + p4) + 10 * (p2 + p3) + 16) >> 5);
+        }
+        inter[x] = v;
+    }
+    __syncthreads();
+    auto s = [&](int i) -> uint32_t { return inter[i]; };
+    auto out = [&](int j) -> uint8_t {
+        if (j >= src_w) return 0;           // beyond the reduced row: never written (zero)
+        if (j >= dst_w) return inter[j];    // vertical-only bytes (super_build.rs:303-305)
+        if (dst_w <= 2) {                   // super_build.rs:314-320
+            uint32_t x0 = (2 * j < src_w) ? s(2 * j) : 0;
+            uint32_t x1 = (2 * j + 1 < src_w) ? s(2 * j + 1) : x0;
+            return avg8(x0, x1);
+        }
+        uint32_t first = avg8(s(0), src_w > 1 ? s(1) : s(0));
+        if (j == 0) return (uint8_t)first;
+        if (j == dst_w - 1) {               // super_build.rs:350-356
+            uint32_t x0 = (2 * j < src_w) ? s(2 * j) : 0;
+            uint32_t x1 = (2 * j + 1 < src_w) ? s(2 * j + 1) : x0;
+            return avg8(x0, x1);
+        }
+        // interior j = i + 1, k = 2j + 3; left = out[0] for j == 1 else s[2j-2] (super_build.rs:327-348)
+        int k = 2 * j + 3;
+        uint32_t left = (j == 1) ? first : s(2 * j - 2);
+        uint32_t pk = (k < src_w) ? s(k) : s(src_w - 1);
+        uint32_t v = (left + pk + 10 * (s(k - 3) + s(k - 2)) + 5 * (s(k - 4) + s(k - 1)) + 16) >> 5;
+        return (uint8_t)min(255u, v);
+    };
+    if ((((uintptr_t)drow) & 3) == 0) {
+        for (int c = threadIdx.x * 4; c < ds; c += kThreads * 4) {
+            if (c + 4 <= ds) {
+                uint32_t wv = (uint32_t)out(c) | ((uint32_t)out(c + 1) << 8) | ((uint32_t)out(c + 2) << 16) |
+                              ((uint32_t)out(c + 3) << 24);
+                *reinterpret_cast<uint32_t*>(drow + c) = wv;
+            } else {
+                for (int k = c; k < ds; k++) drow[k] = out(k);
+            }
+        }
+    } else {
+        for (int c = threadIdx.x; c < ds; c += kThreads) drow[c] = out(c);
+    }
+}
+
+// Levels that the reduce never reaches (scale_down != 4) stay zero, like the
+// reference's pre-cleared plane.
+__global__ void __launch_bounds__(kThreads) super_zero_rows_kernel(SuperGeom g, const Planes* dst, int level) {
+    const int dy = blockIdx.x, plane = blockIdx.y, frame = blockIdx.z;
+    if (dy >= g.level_h[plane][level]) return;
+    const Planes P = dst[frame];
+    uint8_t* row = P.p[plane] + (size_t)(g.level_row[plane][level] + dy) * P.stride[plane];
+    for (int c = threadIdx.x; c < P.stride[plane]; c += kThreads) row[c] = 0;
+}
+
+}  // namespace
+
+hipError_t launch_super_level0(const SuperGeom& g, const Planes* src, const Planes* dst, int nframes,
+                               int max_stride, hipStream_t s) {
+    L0Args a{g, src, dst};
+    dim3 grid(g.level_h[0][0], 3, nframes);
+    size_t lds = 6 * (size_t)(max_stride + 4);
+    hipLaunchKernelGGL(super_level0_kernel, grid, dim3(kThreads), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_super_reduce(const SuperGeom& g, const Planes* dst, int nframes, int level,
+                                       hipStream_t s) {
+    dim3 grid(g.level_h[0][level + 1], 3, nframes);
+    if (g.scale_down == 4) {
+        ReduceArgs a{g, dst, level};
+        size_t lds = (size_t)g.level_w[0][level] + 4;
+        hipLaunchKernelGGL(super_reduce_kernel, grid, dim3(kThreads), lds, s, a);
+    } else {
+        hipLaunchKernelGGL(super_zero_rows_kernel, grid, dim3(kThreads), 0, s, g, dst, level + 1);
+    }
+    return hipGetLastError();
+}
+
+
+}  // namespace svpf

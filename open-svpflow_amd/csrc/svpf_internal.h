@@ -1,0 +1,59 @@
+// Internal declarations shared by the libsvpf translation units (host side and
+// kernel launchers).  Not part of the public C-ABI (include/svpf.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <vector>
+
+#include "../../include/svpf.h"
+
+namespace svpf {
+
+// One YUV420P8 frame's three planes (device or host pointers, byte strides).
+struct Planes {
+    uint8_t* p[3];
+    int stride[3];
+};
+
+// Super-frame geometry for one clip (super_opts.rs:70-95 formulas).
+struct SuperGeom {
+    int width, height;     // luma size of the source
+    int pel, levels;
+    int gpu, scale_down, scale_up, full;
+    int rows[3];           // plane heights of the super frame (luma super_h, chroma super_h/2)
+    int level_row[3][16];  // first row of each level inside each plane
+    int level_w[3][16];    // level width per plane
+    int level_h[3][16];    // level height per plane
+};
+
+int reduce_dim(int dim, int n);
+int auto_levels(int width, int height);
+int super_plane_height(int base_h, int pel, int levels, bool full);
+SuperGeom make_super_geom(const svpf_super_params& p);
+
+// Error reporting for the C-ABI: thread-local last error string.
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define SVPF_HIP(call)                                                                 \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return ::svpf::fail(SVPF_ERR_HIP, std::string(#call " failed: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace svpf
+
+// Stream-ordered upload of a small descriptor table (capi_core.cpp).
+int svpf_upload_desc(svpf_ctx* ctx, const void* data, size_t bytes, void** d_out);
+
+namespace svpf {
+
+// Kernel launchers (super.hip)
+hipError_t launch_super_level0(const SuperGeom& g, const Planes* src, const Planes* dst, int nframes,
+                               int max_stride, hipStream_t s);
+hipError_t launch_super_reduce(const SuperGeom& g, const Planes* dst, int nframes, int level, hipStream_t s);
+
+}  // namespace svpf

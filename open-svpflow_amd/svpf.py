@@ -1,0 +1,207 @@
+"""Python binding of libsvpf (include/svpf.h) over ctypes.
+
+Host-side mirror of the reference's filter interfaces for the motion path
+(svpflow1.Super / svpflow1.Analyse / svpflow2.SmoothFps): the same option strings
+and the same data layouts (super-frame planes, vector payloads, output frames),
+executed by the HIP kernels in libsvpf.so.  There is no CPU fallback: loading
+fails loudly when the library is missing, and every call raises SvpfError when
+the library reports an error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsvpf.so")
+
+
+class SvpfError(RuntimeError):
+    pass
+
+
+class Frame(C.Structure):
+    _fields_ = [("y", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p),
+                ("stride_y", C.c_int32), ("stride_u", C.c_int32), ("stride_v", C.c_int32)]
+
+
+class SuperParams(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("pel", C.c_int32), ("gpu", C.c_int32),
+                ("scale_down", C.c_int32), ("scale_up", C.c_int32), ("full", C.c_int32), ("rc", C.c_int32),
+                ("levels", C.c_int32)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Loads libsvpf.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SvpfError(f"{LIB_PATH} is missing: build it with open-svpflow_amd/build_lib.py")
+        L = C.CDLL(LIB_PATH)
+        L.svpf_last_error.restype = C.c_char_p
+        L.svpf_stream.restype = C.c_void_p
+        L.svpf_super_height.restype = C.c_int32
+        L.svpf_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+        L.svpf_destroy.argtypes = [C.c_void_p]
+        for name in ("svpf_synchronize", "svpf_timer_start"):
+            getattr(L, name).argtypes = [C.c_void_p]
+        L.svpf_timer_stop.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
+        L.svpf_device_alloc.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]
+        L.svpf_device_free.argtypes = [C.c_void_p, C.c_void_p]
+        L.svpf_copy_h2d.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+        L.svpf_copy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+        L.svpf_memset_d.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_size_t]
+        L.svpf_super_params_from_opt.argtypes = [C.c_char_p, C.c_size_t, C.c_int32, C.c_int32,
+                                                 C.POINTER(SuperParams), C.POINTER(C.c_int64), C.c_char_p,
+                                                 C.c_size_t]
+        L.svpf_super_height.argtypes = [C.POINTER(SuperParams)]
+        L.svpf_super_build.argtypes = [C.c_void_p, C.POINTER(SuperParams), C.POINTER(Frame), C.POINTER(Frame)]
+        L.svpf_super_build_batch.argtypes = [C.c_void_p, C.POINTER(SuperParams), C.c_int32, C.POINTER(Frame),
+                                             C.POINTER(Frame)]
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise SvpfError(f"libsvpf error {rc}: {lib().svpf_last_error().decode(errors='replace')}")
+
+
+def exported_symbols() -> list[str]:
+    """Names declared in include/svpf.h (used by the symbol-export test)."""
+    import re
+    path = os.path.join(HERE, "..", "include", "svpf.h")
+    text = open(path).read()
+    return sorted(set(re.findall(r"\b(svpf_[a-z0-9_]+)\s*\(", text)))
+
+
+class Context:
+    """A libsvpf context: one device, one HIP stream."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().svpf_create(device, C.byref(h)))
+        self.h = h
+        self._allocs: list[int] = []
+
+    def close(self) -> None:
+        if self.h:
+            for p in self._allocs:
+                lib().svpf_device_free(self.h, p)
+            self._allocs.clear()
+            lib().svpf_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- memory
+    def alloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        check(lib().svpf_device_alloc(self.h, nbytes, C.byref(p)))
+        self._allocs.append(p.value)
+        return p.value
+
+    def free(self, ptr: int) -> None:
+        if ptr in self._allocs:
+            self._allocs.remove(ptr)
+        check(lib().svpf_device_free(self.h, ptr))
+
+    def upload(self, dptr: int, arr: np.ndarray) -> None:
+        a = np.ascontiguousarray(arr)
+        check(lib().svpf_copy_h2d(self.h, dptr, a.ctypes.data, a.nbytes))
+
+    def download(self, dptr: int, nbytes: int, dtype=np.uint8) -> np.ndarray:
+        out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype=dtype)
+        check(lib().svpf_copy_d2h(self.h, out.ctypes.data, dptr, out.nbytes))
+        return out
+
+    def memset(self, dptr: int, value: int, nbytes: int) -> None:
+        check(lib().svpf_memset_d(self.h, dptr, value, nbytes))
+
+    def sync(self) -> None:
+        check(lib().svpf_synchronize(self.h))
+
+    def timer_start(self) -> None:
+        check(lib().svpf_timer_start(self.h))
+
+    def timer_stop(self) -> float:
+        ms = C.c_float()
+        check(lib().svpf_timer_stop(self.h, C.byref(ms)))
+        return ms.value
+
+
+@dataclass
+class DeviceFrame:
+    """A YUV420P8 frame in device memory: three planes, each `rows` x `stride`."""
+    ctx: Context
+    y: int
+    u: int
+    v: int
+    strides: tuple
+    rows: tuple
+
+    @classmethod
+    def alloc(cls, ctx: Context, strides, rows) -> "DeviceFrame":
+        ptrs = [ctx.alloc(s * r) for s, r in zip(strides, rows)]
+        return cls(ctx, ptrs[0], ptrs[1], ptrs[2], tuple(strides), tuple(rows))
+
+    def desc(self) -> Frame:
+        return Frame(self.y, self.u, self.v, *self.strides)
+
+    def upload(self, planes) -> None:
+        for ptr, stride, rows, p in zip((self.y, self.u, self.v), self.strides, self.rows, planes):
+            buf = np.zeros((rows, stride), np.uint8)
+            buf[:p.shape[0], :p.shape[1]] = p
+            self.ctx.upload(ptr, buf)
+
+    def download(self):
+        return [self.ctx.download(ptr, s * r).reshape(r, s)
+                for ptr, s, r in zip((self.y, self.u, self.v), self.strides, self.rows)]
+
+
+# ---------------------------------------------------------------- Super
+
+def super_params(opt: str | bytes | None, width: int, height: int):
+    """SuperOpts::from_opt (super_opts.rs:17-49) -> (SuperParams, sdata)."""
+    p = SuperParams()
+    sdata = C.c_int64()
+    err = C.create_string_buffer(256)
+    b = opt.encode() if isinstance(opt, str) else (opt or b"")
+    rc = lib().svpf_super_params_from_opt(b, len(b), width, height, C.byref(p), C.byref(sdata), err, 256)
+    if rc == -3:
+        raise SvpfError(err.value.decode())
+    check(rc)
+    return p, sdata.value
+
+
+def super_height(p: SuperParams) -> int:
+    return lib().svpf_super_height(C.byref(p))
+
+
+def super_build(ctx: Context, p: SuperParams, src: DeviceFrame, dst: DeviceFrame) -> None:
+    """svpflow1.Super for one frame: d_src -> d_dst (all planes written)."""
+    s, d = src.desc(), dst.desc()
+    check(lib().svpf_super_build(ctx.h, C.byref(p), C.byref(s), C.byref(d)))
+
+
+def super_build_batch(ctx: Context, p: SuperParams, srcs, dsts) -> None:
+    n = len(srcs)
+    S = (Frame * n)(*[f.desc() for f in srcs])
+    D = (Frame * n)(*[f.desc() for f in dsts])
+    check(lib().svpf_super_build_batch(ctx.h, C.byref(p), n, S, D))

@@ -1,0 +1,39 @@
+"""The C-ABI library loads and exports every symbol include/svpf.h declares (no GPU calls)."""
+import ctypes
+import os
+
+import svpf
+
+
+def test_library_exports_declared_symbols():
+    names = svpf.exported_symbols()
+    assert len(names) >= 10
+    lib = ctypes.CDLL(svpf.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, f"declared but not exported: {missing}"
+
+
+def test_option_parsing_matches_oracle(oracle):
+    # product-side SuperOpts::from_opt vs the oracle's restatement (host logic, no GPU)
+    for opt in (None, "{pel:1}", "{pel:4, gpu:0}", "{gpu:2}", "{pel:2, full:1}", "{pel:2, rc:true}",
+                "{ /* c */ pel : 1 // x\n }", "{pel:2.7}", "{pel:-}", "garbage", "{pel:2, pel:1}"):
+        for w, h in ((1920, 1080), (640, 360), (854, 480), (34, 18)):
+            o, err = oracle.super_opts(opt, w, h)
+            try:
+                p, sdata = svpf.super_params(opt, w, h)
+            except svpf.SvpfError as e:
+                assert o is None and str(e) == err
+                continue
+            assert o is not None, opt
+            assert (p.width, p.height, p.pel, p.gpu, p.scale_down, p.scale_up, p.full, p.rc, p.levels) == \
+                tuple(o[k] for k in ("width", "height", "pel", "gpu", "scale_down", "scale_up", "full", "rc",
+                                     "levels"))
+            assert sdata == o["sdata"]
+            assert svpf.super_height(p) == o["super_h"]
+    for opt in ("{pel:3}", "{gpu:3}", "{full:false}"):
+        o, err = oracle.super_opts(opt, 64, 64)
+        try:
+            svpf.super_params(opt, 64, 64)
+            assert False, "expected an error"
+        except svpf.SvpfError as e:
+            assert str(e) == err
