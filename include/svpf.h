@@ -89,6 +89,50 @@ int svpf_super_build(svpf_ctx* ctx, const svpf_super_params* p, const svpf_frame
 int svpf_super_build_batch(svpf_ctx* ctx, const svpf_super_params* p, int32_t n, const svpf_frame* d_src,
                            const svpf_frame* d_dst);
 
+/* ------------------------------------------------------------------ Analyse */
+/* Replaces AnalyseOpts (crates/svpflow1/src/analyse_opts.rs:4-240). */
+typedef struct svpf_analyse_params {
+    int32_t vectors;            /* 1 bwd, 2 fwd, 3 both */
+    int32_t block_w, block_h;
+    int32_t overlap_mode, overlap_x, overlap_y;
+    int32_t delta;
+    int32_t pel, selector, levels, super_levels;
+    int32_t width, height;
+    int32_t lambda, lsad;
+    double plevel;
+    int32_t pnew, pglobal, pzero, pnbour, prev;
+    int32_t search_type, coarse_search_type, distance, coarse_distance;
+    int32_t coarse_trymany, coarse_bad_sad, coarse_bad_range;
+    int32_t fine_satd, coarse_satd;
+    int32_t has_refine, refine_thsad, refine_search_type, refine_distance, refine_satd;
+    int32_t sort;
+} svpf_analyse_params;
+
+/* AnalyseOpts::from_opt (analyse_opts.rs:45-219, sdata unpacked as :243-257), with
+ * width/height overridden by the source clip when > 0 (analyse_filter.rs:92-97).
+ * vdata16 = pack_vdata_header (analyse_search.rs:2813-2841); *payload_len =
+ * the vector-clip width (analyse_filter.rs:122-129). */
+int svpf_analyse_params_from_opt(const char* opt, size_t opt_len, int64_t sdata, int32_t width, int32_t height,
+                                 svpf_analyse_params* out, int32_t* vdata16, int32_t* payload_len, char* err,
+                                 size_t err_len);
+
+/* analyse_pair + pack_vector_frame for one frame pair (analyse_search.rs:68-92,
+ * 2740-2811, as called by get_frame_analyse, analyse_filter.rs:229-334):
+ * d_cur = super frame n, d_ref = super frame n+delta (device), d_payload = device
+ * buffer of payload_len bytes (4-byte aligned) receiving the vector frame. */
+int svpf_analyse_pair(svpf_ctx* ctx, const svpf_analyse_params* p, const svpf_frame* d_cur, const svpf_frame* d_ref,
+                      uint8_t* d_payload);
+
+/* Batched form: n pairs of one clip searched together (one wavefront per pair and
+ * direction, one launch per pyramid level and stage).  Arrays hold n entries. */
+int svpf_analyse_batch(svpf_ctx* ctx, const svpf_analyse_params* p, int32_t n, const svpf_frame* d_cur,
+                       const svpf_frame* d_ref, uint8_t* const* d_payload);
+
+/* Multi-clip batch: pair i uses params[i] (clips of different sizes and pyramid
+ * depths share every launch; levels are aligned at level 0). */
+int svpf_analyse_multi(svpf_ctx* ctx, int32_t n, const svpf_analyse_params* const* params, const svpf_frame* d_cur,
+                       const svpf_frame* d_ref, uint8_t* const* d_payload);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,97 @@
+// Device-side data layouts of the Analyse search (shared by analyse.hip and the
+// host orchestration in analyse_host.cpp).
+#pragma once
+#include <cstdint>
+
+namespace svpf {
+
+// One block vector of a level field (analyse_search.rs:25-31 Vec8 plus a
+// "searched" flag that replaces the reference's `processed` grid).  16 bytes.
+struct alignas(16) DVec {
+    int16_t dx, dy;
+    uint32_t score;
+    uint32_t luma;
+    uint32_t done;
+};
+
+// Planes of one super frame (device pointers, byte strides, byte lengths).
+struct SPlanes {
+    const uint8_t* p[3];
+    int32_t stride[3];
+    int64_t len[3];
+};
+
+// Per-level geometry and options shared by all chains of one clip.
+struct LevelGeom {
+    int32_t level, gw, gh;
+    int32_t lw, lh;        // luma level size
+    int32_t y_off;         // first luma row of the level (level_y_offset)
+    int32_t c_off;         // first chroma row of the level (chroma_level_offset)
+    int32_t clw, clh;      // chroma level size
+    int32_t pel;           // pel of this level (opts.pel at level 0, else 1)
+    int32_t bw, bh;        // opts block size
+    int32_t step_x, step_y;
+    int32_t smallest;      // level == levels-1 (half-size centred block, no chroma)
+    int32_t satd;
+    int32_t search_type;
+    int32_t radius;
+    int32_t lambda_base;   // exact_lambda(level) (0 at the smallest level)
+    int32_t lsad;          // opts.lsad (lsad/3 at the smallest level)
+    int32_t pnew, pglobal, pzero, pnbour, pprev;
+    int32_t early;         // level == 0 && distance == 0: predictor only
+    int32_t bad_sad, bad_range;
+    int32_t adaptive;      // coarse distance < 0: adaptive radius from block activity
+};
+
+// One (pair, direction) search chain at one level.
+struct Chain {
+    SPlanes src;           // frame whose blocks are searched
+    SPlanes ref;           // reference frame
+    DVec* field;           // in: interpolated predictors; out: searched vectors
+    const DVec* prev;      // projected opposite-direction field (nullptr at the top level)
+    const int32_t* order;  // visit order, packed x | y << 16
+    const LevelGeom* geom;
+    const int32_t* global; // (gx, gy) global predictor from the coarser level; nullptr = (0, 0)
+};
+
+
+// Level-transition and packing job descriptors (one per chain).
+struct Transition {
+    const DVec* coarse;     // searched field of level l+1 (cgw x cgh)
+    DVec* next;             // field of level l (gw x gh), written
+    unsigned long long* proj_key;  // gw*gh projection keys (score << 32 | source raster index)
+    DVec* proj;             // projected field for the opposite direction of level l
+    int cgw, cgh, gw, gh;
+    int bw, bh, ox, oy, pel;
+};
+
+struct GlobalJob {
+    const DVec* field;
+    int n;
+    int lw, lh;     // bounds of |dx|, |dy| (level size)
+    int* out;       // (gx, gy)
+};
+
+struct SortJob {
+    const DVec* field;     // level l+1 field (gw x gh)
+    uint32_t* keys;        // (gw-1)*(gh-1)
+    uint32_t* vals;
+    int gw, gh, shift;
+    int seg_begin;         // offset of this chain's segment in the concatenated arrays
+};
+
+struct OrderJob {
+    const uint32_t* parents;  // sorted parent positions (x | y << 16), count = (cgw-1)*(cgh-1)
+    int32_t* order;           // order of level l (gw*gh entries)
+    int cgw, cgh, gw, gh;
+};
+
+struct PackJob {
+    const DVec* field;      // final level-0 field
+    uint8_t* out;           // payload region start (marker position)
+    uint8_t* payload;       // payload start: receives the 64-byte header
+    const int32_t* header;  // 16 header words
+    int count;
+};
+
+}  // namespace svpf

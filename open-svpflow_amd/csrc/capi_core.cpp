@@ -5,17 +5,6 @@
 #include "options.h"
 #include "svpf_internal.h"
 
-struct svpf_ctx {
-    int device;
-    hipStream_t stream;
-    hipEvent_t t0, t1;
-    // Descriptor arrays for launches: pinned staging buffer + device copy, grown on
-    // demand.  `desc_done` marks when the last upload finished reading the staging buffer.
-    void* h_desc = nullptr;
-    void* d_desc = nullptr;
-    size_t desc_cap = 0;
-    hipEvent_t desc_done = nullptr;
-};
 
 namespace svpf {
 

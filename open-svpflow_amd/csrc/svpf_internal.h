@@ -9,6 +9,23 @@
 
 #include "../../include/svpf.h"
 
+struct svpf_ctx {
+    int device;
+    hipStream_t stream;
+    hipEvent_t t0, t1;
+    // Descriptor arrays for launches: pinned staging buffer + device copy, grown on
+    // demand.  `desc_done` marks when the last upload finished reading the staging buffer.
+    void* h_desc = nullptr;
+    void* d_desc = nullptr;
+    size_t desc_cap = 0;
+    hipEvent_t desc_done = nullptr;
+    // Analyse workspace (device), grown on demand and reused across calls.
+    void* d_work = nullptr;
+    size_t work_cap = 0;
+    void* d_sort_tmp = nullptr;
+    size_t sort_tmp_cap = 0;
+};
+
 namespace svpf {
 
 // One YUV420P8 frame's three planes (device or host pointers, byte strides).

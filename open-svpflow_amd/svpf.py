@@ -35,6 +35,7 @@ class SuperParams(C.Structure):
 
 
 _lib = None
+_BINDERS = []  # per-section argtype setup, run once when the library loads
 
 
 def lib() -> C.CDLL:
@@ -64,6 +65,8 @@ def lib() -> C.CDLL:
         L.svpf_super_build.argtypes = [C.c_void_p, C.POINTER(SuperParams), C.POINTER(Frame), C.POINTER(Frame)]
         L.svpf_super_build_batch.argtypes = [C.c_void_p, C.POINTER(SuperParams), C.c_int32, C.POINTER(Frame),
                                              C.POINTER(Frame)]
+        for bind in _BINDERS:
+            bind(L)
         _lib = L
     return _lib
 
@@ -205,3 +208,68 @@ def super_build_batch(ctx: Context, p: SuperParams, srcs, dsts) -> None:
     S = (Frame * n)(*[f.desc() for f in srcs])
     D = (Frame * n)(*[f.desc() for f in dsts])
     check(lib().svpf_super_build_batch(ctx.h, C.byref(p), n, S, D))
+
+
+# ---------------------------------------------------------------- Analyse
+
+class AnalyseParams(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "vectors", "block_w", "block_h", "overlap_mode", "overlap_x", "overlap_y", "delta", "pel", "selector",
+        "levels", "super_levels", "width", "height", "lambda_", "lsad")] + [("plevel", C.c_double)] + [
+        (n, C.c_int32) for n in (
+            "pnew", "pglobal", "pzero", "pnbour", "prev", "search_type", "coarse_search_type", "distance",
+            "coarse_distance", "coarse_trymany", "coarse_bad_sad", "coarse_bad_range", "fine_satd", "coarse_satd",
+            "has_refine", "refine_thsad", "refine_search_type", "refine_distance", "refine_satd", "sort")]
+
+
+def _bind_analyse(L):
+    L.svpf_analyse_params_from_opt.argtypes = [C.c_char_p, C.c_size_t, C.c_int64, C.c_int32, C.c_int32,
+                                               C.POINTER(AnalyseParams), C.POINTER(C.c_int32),
+                                               C.POINTER(C.c_int32), C.c_char_p, C.c_size_t]
+    L.svpf_analyse_pair.argtypes = [C.c_void_p, C.POINTER(AnalyseParams), C.POINTER(Frame), C.POINTER(Frame),
+                                    C.c_void_p]
+    L.svpf_analyse_batch.argtypes = [C.c_void_p, C.POINTER(AnalyseParams), C.c_int32, C.POINTER(Frame),
+                                     C.POINTER(Frame), C.POINTER(C.c_void_p)]
+    L.svpf_analyse_multi.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.POINTER(AnalyseParams)),
+                                     C.POINTER(Frame), C.POINTER(Frame), C.POINTER(C.c_void_p)]
+
+
+_BINDERS.append(_bind_analyse)
+
+
+def analyse_params(opt, sdata: int, width: int = 0, height: int = 0):
+    """AnalyseOpts::from_opt -> (AnalyseParams, vdata[16], payload_len)."""
+    L = lib()
+    p = AnalyseParams()
+    vd = (C.c_int32 * 16)()
+    plen = C.c_int32()
+    err = C.create_string_buffer(256)
+    b = opt.encode() if isinstance(opt, str) else (opt or b"")
+    rc = L.svpf_analyse_params_from_opt(b, len(b), sdata, width, height, C.byref(p), vd, C.byref(plen), err, 256)
+    if rc == -3:
+        raise SvpfError(err.value.decode())
+    check(rc)
+    return p, list(vd), plen.value
+
+
+def analyse_pair(ctx: Context, p: AnalyseParams, cur: DeviceFrame, ref: DeviceFrame, d_payload: int) -> None:
+    """svpflow1.Analyse for one pair of super frames -> vector payload in device memory."""
+    c, r = cur.desc(), ref.desc()
+    check(lib().svpf_analyse_pair(ctx.h, C.byref(p), C.byref(c), C.byref(r), C.c_void_p(d_payload)))
+
+
+def analyse_batch(ctx: Context, p: AnalyseParams, curs, refs, d_payloads) -> None:
+    n = len(curs)
+    Cu = (Frame * n)(*[f.desc() for f in curs])
+    R = (Frame * n)(*[f.desc() for f in refs])
+    P = (C.c_void_p * n)(*d_payloads)
+    check(lib().svpf_analyse_batch(ctx.h, C.byref(p), n, Cu, R, P))
+
+
+def analyse_multi(ctx: Context, params, curs, refs, d_payloads) -> None:
+    n = len(curs)
+    PP = (C.POINTER(AnalyseParams) * n)(*[C.pointer(p) for p in params])
+    Cu = (Frame * n)(*[f.desc() for f in curs])
+    R = (Frame * n)(*[f.desc() for f in refs])
+    P = (C.c_void_p * n)(*d_payloads)
+    check(lib().svpf_analyse_multi(ctx.h, n, PP, Cu, R, P))

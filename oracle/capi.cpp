@@ -97,3 +97,127 @@ int oracle_super_build(const char* opt, size_t len, int w, int h, const uint8_t*
 }
 
 }  // extern "C"
+
+#include "analyse.h"
+
+extern "C" {
+
+// AnalyseOpts as seen by the vector clip: returns 0 or -3 with the error text.
+// out = {vectors, block_w, block_h, overlap_x, overlap_y, pel, levels, super_levels, width, height,
+//        lambda, lsad, search_type, coarse_search_type, distance, payload_len}
+int oracle_analyse_opts(const char* opt, size_t len, int64_t sdata, int width, int height, int32_t* out,
+                        int32_t* vdata16, char* err, size_t errlen) {
+    Value v;
+    bool have = parse_opt(opt, len, &v);
+    AnalyseOpts o;
+    std::string e = analyse_opts_from(have ? &v : nullptr, sdata, nullptr, &o);
+    if (!e.empty()) {
+        put_err(err, errlen, e);
+        return -3;
+    }
+    if (width > 0) o.width = width;
+    if (height > 0) o.height = height;
+    int bw, bh, ox, oy, gw, gh;
+    o.output_block(&bw, &bh, &ox, &oy);
+    o.grid(bw, bh, ox, oy, &gw, &gh);
+    int nreg = (o.vectors == 1 || o.vectors == 2) ? 1 : 2;
+    int32_t vals[16] = {o.vectors, o.block_w, o.block_h, o.overlap_x, o.overlap_y, o.pel, o.levels, o.super_levels,
+                        o.width, o.height, o.lambda, o.lsad, o.search_type, o.coarse_search_type, o.distance,
+                        0x40 + nreg * (4 + gw * gh * 8)};
+    std::memcpy(out, vals, sizeof vals);
+    auto h = pack_vdata_header(o);
+    std::memcpy(vdata16, h.data(), 16 * 4);
+    return 0;
+}
+
+// get_frame_analyse (analyse_filter.rs:229-334) on two super frames held in host
+// memory: cur = super frame n, ref = super frame n+delta.  Writes the vector payload.
+int oracle_analyse(const char* opt, size_t len, int64_t sdata, int width, int height, const uint8_t* cy, int cys,
+                   const uint8_t* cu, int cus, const uint8_t* cv, int cvs, const uint8_t* ry, int rys,
+                   const uint8_t* ru, int rus, const uint8_t* rv, int rvs, uint8_t* out, size_t out_cap,
+                   size_t* out_len, uint64_t* evals16, uint64_t* pixels16, int threads) {
+    Value v;
+    bool have = parse_opt(opt, len, &v);
+    AnalyseOpts o;
+    if (!analyse_opts_from(have ? &v : nullptr, sdata, nullptr, &o).empty()) return -3;
+    if (width > 0) o.width = width;
+    if (height > 0) o.height = height;
+    size_t yh = (size_t)super_plane_height(o.height, o.pel, o.super_levels, true), uvh = yh / 2;
+    SuperPlanes c{cy, (size_t)cys * yh, (size_t)cys, cu, (size_t)cus * uvh, (size_t)cus, cv, (size_t)cvs * uvh,
+                  (size_t)cvs, (size_t)o.width, (size_t)o.height, o.pel, o.super_levels, true};
+    SuperPlanes r{ry, (size_t)rys * yh, (size_t)rys, ru, (size_t)rus * uvh, (size_t)rus, rv, (size_t)rvs * uvh,
+                  (size_t)rvs, (size_t)o.width, (size_t)o.height, o.pel, o.super_levels, true};
+    bool hb, hf;
+    std::vector<Vec8> bwd, fwd;
+    SearchCounters sc;
+    analyse_pair(c, r, o, &hb, &bwd, &hf, &fwd, &sc, threads != 0);
+    auto payload = pack_vector_frame(o, hb ? &bwd : nullptr, hf ? &fwd : nullptr);
+    *out_len = payload.size();
+    if (payload.size() > out_cap) return -1;
+    std::memcpy(out, payload.data(), payload.size());
+    if (evals16) std::memcpy(evals16, sc.evals, sizeof sc.evals);
+    if (pixels16) std::memcpy(pixels16, sc.pixels, sizeof sc.pixels);
+    return 0;
+}
+
+// Single candidate cost for focused tests: (luma, chroma) by the reference's path selection.
+int oracle_candidate_cost(int width, int height, int pel, int super_levels, const uint8_t* cy, int cys,
+                          const uint8_t* cu, int cus, const uint8_t* cv, int cvs, const uint8_t* ry, int rys,
+                          const uint8_t* ru, int rus, const uint8_t* rv, int rvs, int level, int px, int py, int bw,
+                          int bh, int mvx, int mvy, int satd, int chroma, uint32_t* luma, uint32_t* colour) {
+    size_t yh = (size_t)super_plane_height(height, pel, super_levels, true), uvh = yh / 2;
+    SuperPlanes c{cy, (size_t)cys * yh, (size_t)cys, cu, (size_t)cus * uvh, (size_t)cus, cv, (size_t)cvs * uvh,
+                  (size_t)cvs, (size_t)width, (size_t)height, pel, super_levels, true};
+    SuperPlanes r{ry, (size_t)rys * yh, (size_t)rys, ru, (size_t)rus * uvh, (size_t)rus, rv, (size_t)rvs * uvh,
+                  (size_t)rvs, (size_t)width, (size_t)height, pel, super_levels, true};
+    candidate_cost(c, r, level, px, py, bw, bh, mvx, mvy, satd != 0, level == 0 ? pel : 1, chroma != 0, luma, colour);
+    return 0;
+}
+
+uint32_t oracle_hadamard4(const int32_t* d16) {
+    int32_t d[4][4];
+    std::memcpy(d, d16, sizeof d);
+    return hadamard4_satd(d);
+}
+
+int oracle_level_order(int gw, int gh, int level, int32_t* xy, int cap) {
+    auto o = level_search_order(gw, gh, level);
+    int n = (int)o.size();
+    for (int i = 0; i < n && i < cap; i++) { xy[2 * i] = o[i].first; xy[2 * i + 1] = o[i].second; }
+    return n;
+}
+
+int oracle_lambdas(int lambda, double plevel, int levels, int level, int pel, int lsad, uint32_t score,
+                   int32_t* exact, int32_t* adapted) {
+    AnalyseOpts o;
+    o.lambda = lambda;
+    o.plevel = plevel;
+    o.levels = levels;
+    *exact = exact_lambda(o, level, pel);
+    *adapted = adapt_lambda(*exact, lsad, score);
+    return 0;
+}
+
+int oracle_global(const int16_t* dxdy, int n, int32_t* gx, int32_t* gy) {
+    std::vector<Vec8> f((size_t)n);
+    for (int i = 0; i < n; i++) { f[(size_t)i].dx = dxdy[2 * i]; f[(size_t)i].dy = dxdy[2 * i + 1]; }
+    int x, y;
+    exact_global_doubled(f, &x, &y);
+    *gx = x;
+    *gy = y;
+    return 0;
+}
+
+int oracle_rescale(int16_t* dxdy, uint32_t* score, int n, int width, int height, int blk) {
+    std::vector<Vec8> f((size_t)n);
+    for (int i = 0; i < n; i++) {
+        f[(size_t)i].dx = dxdy[2 * i];
+        f[(size_t)i].dy = dxdy[2 * i + 1];
+        f[(size_t)i].score = score[i];
+    }
+    rescale_scores(f, width, height, blk);
+    for (int i = 0; i < n; i++) score[i] = f[(size_t)i].score;
+    return 0;
+}
+
+}  // extern "C"

@@ -78,3 +78,91 @@ def super_build(opt: str | None, planes, strides=None, dst_strides=None):
                               dst_strides[2])
     assert rc == 0
     return dst
+
+_sz = C.c_size_t
+L.oracle_analyse_opts.argtypes = [C.c_char_p, _sz, C.c_int64, _i, _i, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                  C.c_char_p, _sz]
+L.oracle_analyse.argtypes = [C.c_char_p, _sz, C.c_int64, _i, _i] + [u8p, _i] * 6 + [
+    u8p, _sz, C.POINTER(_sz), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), _i]
+L.oracle_candidate_cost.argtypes = [_i, _i, _i, _i] + [u8p, _i] * 6 + [_i] * 9 + [C.POINTER(C.c_uint32)] * 2
+L.oracle_hadamard4.argtypes = [C.POINTER(C.c_int32)]
+L.oracle_hadamard4.restype = C.c_uint32
+L.oracle_level_order.argtypes = [_i, _i, _i, C.POINTER(C.c_int32), _i]
+L.oracle_lambdas.argtypes = [_i, C.c_double, _i, _i, _i, _i, C.c_uint32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+L.oracle_global.argtypes = [C.POINTER(C.c_int16), _i, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+L.oracle_rescale.argtypes = [C.POINTER(C.c_int16), C.POINTER(C.c_uint32), _i, _i, _i, _i]
+
+ANALYSE_KEYS = ("vectors", "block_w", "block_h", "overlap_x", "overlap_y", "pel", "levels", "super_levels", "width",
+                "height", "lambda", "lsad", "search_type", "coarse_search_type", "distance", "payload_len")
+
+
+def analyse_opts(opt: str | None, sdata: int, width: int = 0, height: int = 0):
+    b = (opt or "").encode()
+    out = (C.c_int32 * 16)()
+    vd = (C.c_int32 * 16)()
+    err = C.create_string_buffer(256)
+    rc = L.oracle_analyse_opts(b, len(b), sdata, width, height, out, vd, err, 256)
+    if rc:
+        return None, err.value.decode()
+    d = dict(zip(ANALYSE_KEYS, list(out)))
+    d["vdata"] = list(vd)
+    return d, None
+
+
+def analyse(opt: str | None, sdata: int, width: int, height: int, cur, ref, threads: bool = False,
+            counters: bool = False):
+    """get_frame_analyse on two super frames (lists of 3 planes rows x stride) -> payload bytes."""
+    b = (opt or "").encode()
+    o, err = analyse_opts(opt, sdata, width, height)
+    if o is None:
+        raise ValueError(err)
+    out = np.zeros(o["payload_len"] + 64, np.uint8)
+    n = _sz()
+    ev = (C.c_uint64 * 16)()
+    px = (C.c_uint64 * 16)()
+    args = []
+    for planes in (cur, ref):
+        for p in planes:
+            args += [_p(np.ascontiguousarray(p)), p.shape[1]]
+    keep = [np.ascontiguousarray(p) for p in list(cur) + list(ref)]
+    args = []
+    for p in keep:
+        args += [_p(p), p.shape[1]]
+    rc = L.oracle_analyse(b, len(b), sdata, width, height, *args, _p(out), out.size, C.byref(n), ev, px,
+                          1 if threads else 0)
+    assert rc == 0, rc
+    payload = out[:n.value].copy()
+    if counters:
+        return payload, list(ev), list(px)
+    return payload
+
+
+def candidate_cost(width, height, pel, super_levels, cur, ref, level, px, py, bw, bh, mvx, mvy, satd, chroma):
+    keep = [np.ascontiguousarray(p) for p in list(cur) + list(ref)]
+    args = []
+    for p in keep:
+        args += [_p(p), p.shape[1]]
+    l = C.c_uint32()
+    c = C.c_uint32()
+    L.oracle_candidate_cost(width, height, pel, super_levels, *args, level, px, py, bw, bh, mvx, mvy, int(satd),
+                            int(chroma), C.byref(l), C.byref(c))
+    return l.value, c.value
+
+
+def decode_payload(payload: np.ndarray):
+    """Vector payload -> (header[16], [regions]) with each region an (N, 4) array dx, dy, score, luma."""
+    hdr = payload[:64].view(np.int32).copy()
+    regs = []
+    off = 0x40
+    n_reg = (1 if hdr[2] & 1 else 0) + (1 if hdr[2] & 2 else 0)
+    count = int(hdr[12]) * int(hdr[13])
+    for _ in range(n_reg):
+        marker = payload[off:off + 4].view(np.int32)[0]
+        assert marker == 2 * count + 1
+        d = payload[off + 4:off + 4 + 8 * count].view(np.uint32).reshape(count, 2)
+        dx = (d[:, 0] >> 16).astype(np.uint16).view(np.int16)
+        dy = (d[:, 0] & 0xFFFF).astype(np.uint16).view(np.int16)
+        regs.append(np.stack([dx.astype(np.int64), dy.astype(np.int64), (d[:, 1] & 0xFFFFFF).astype(np.int64),
+                              (d[:, 1] >> 24).astype(np.int64)], 1))
+        off += 4 + 8 * count
+    return hdr, regs

@@ -1,0 +1,105 @@
+"""GPU parity of svpflow1.Analyse: libsvpf's HIP search vs the CPU oracle.
+
+Vectors, costs and luma DC of every block (the whole vector payload) must be
+byte-identical, for the benchmark settings (hex2/SAD 16x16, UMH/SATD 8x8 with
+overlap, exhaustive pel 1) and for option paths that change the search
+(single direction, distance 0, non-square blocks, negative pnew, positive
+bad-block range, adaptive coarse radius, padded strides).
+"""
+import numpy as np
+import pytest
+
+import svpf
+import synth
+
+pytestmark = pytest.mark.gpu
+
+CFG1 = "{block:{w:16,h:16,overlap:0},main:{search:{type:2,distance:8,satd:false,coarse:{satd:false}}}}"
+CFG2 = "{block:{w:8,h:8,overlap:3},main:{levels:4,search:{type:3,distance:16,satd:true}}}"
+CFG3 = "{block:{w:16,h:16,overlap:3},main:{levels:5,search:{type:4,distance:8,satd:false,coarse:{satd:false}}}}"
+
+
+def _supers(ctx, oracle, frames, sopt, pad=0):
+    h, w = frames[0][0].shape
+    p, sdata = svpf.super_params(sopt, w, h)
+    sh = svpf.super_height(p)
+    strides = (w + pad, w // 2 + pad, w // 2 + pad)
+    devs, hosts = [], []
+    for f in frames:
+        src = svpf.DeviceFrame.alloc(ctx, strides, (h, h // 2, h // 2))
+        src.upload(f)
+        dst = svpf.DeviceFrame.alloc(ctx, strides, (sh, sh // 2, sh // 2))
+        svpf.super_build(ctx, p, src, dst)
+        devs.append(dst)
+    ctx.sync()
+    for d in devs:
+        hosts.append(d.download())
+    return sdata, devs, hosts
+
+
+def _check(ctx, oracle, frames, sopt, aopt, pad=0, batch=False):
+    h, w = frames[0][0].shape
+    sdata, devs, hosts = _supers(ctx, oracle, frames, sopt, pad)
+    p, vdata, plen = svpf.analyse_params(aopt, sdata, w, h)
+    pays = [ctx.alloc(plen) for _ in range(len(frames) - 1)]
+    if batch:
+        svpf.analyse_batch(ctx, p, devs[:-1], devs[1:], pays)
+    else:
+        for i in range(len(frames) - 1):
+            svpf.analyse_pair(ctx, p, devs[i], devs[i + 1], pays[i])
+    ctx.sync()
+    for i in range(len(frames) - 1):
+        got = ctx.download(pays[i], plen)
+        ref = oracle.analyse(aopt, sdata, w, h, hosts[i], hosts[i + 1])
+        assert got.size == ref.size
+        if not np.array_equal(got, ref):
+            hg, rg = oracle.decode_payload(got)
+            hr, rr = oracle.decode_payload(ref)
+            assert (hg == hr).all(), f"header differs: {hg} vs {hr}"
+            for k, (a, b) in enumerate(zip(rg, rr)):
+                bad = np.argwhere((a != b).any(1))[:, 0]
+                assert bad.size == 0, (f"pair {i} region {k}: {bad.size} blocks differ; first {bad[:4].tolist()} "
+                                       f"got {a[bad[:4]].tolist()} want {b[bad[:4]].tolist()}")
+    for d in pays:
+        ctx.free(d)
+
+
+@pytest.mark.parametrize("aopt", [CFG1, None])
+def test_analyse_cfg1_settings(gpu_ctx, oracle, aopt):
+    print(f"analyse parity: cfg1 {aopt}", flush=True)
+    frames = synth.config_clip(1, frames=3)
+    _check(gpu_ctx, oracle, frames, "{pel:2}", aopt)
+
+
+def test_analyse_cfg2_settings_small(gpu_ctx, oracle):
+    print("analyse parity: cfg2 settings 480x272", flush=True)
+    frames = synth.clip(480, 272, 3, seed=2024, vmax=12.0, nobj=(3, 5), rot_deg=2.0)
+    _check(gpu_ctx, oracle, frames, "{pel:2}", CFG2)
+
+
+def test_analyse_cfg3_settings_small(gpu_ctx, oracle):
+    print("analyse parity: cfg3 settings 512x288 pel1", flush=True)
+    frames = synth.clip(512, 288, 2, seed=3033, vmax=10.0, nobj=(3, 5), rot_deg=2.0)
+    _check(gpu_ctx, oracle, frames, "{pel:1}", CFG3)
+
+
+@pytest.mark.parametrize("aopt", [
+    "{vectors:1, block:{w:8,h:8,overlap:2}, main:{search:{type:3,distance:8,satd:true}}}",
+    "{vectors:2, block:{w:16,h:8,overlap:1}, main:{search:{type:2,distance:6}}}",
+    "{block:{w:8,h:8,overlap:0}, main:{search:{type:4,distance:0}}}",
+    "{block:{w:8,h:8,overlap:3}, main:{penalty:{pnew:-300, pzero:10}, search:{type:2,distance:6}}}",
+    "{block:{w:16,h:16,overlap:2}, main:{search:{type:3,distance:12,coarse:{bad:{range:6, sad:200}}}}}",
+    "{block:{w:8,h:8,overlap:3}, main:{search:{type:2,distance:8,coarse:{distance:-6, satd:false}}}}",
+    "{block:{w:4,h:4,overlap:0}, main:{search:{type:2,distance:4,satd:true}}}",
+    "{block:{w:32,h:32,overlap:3}, main:{search:{type:3,distance:8,satd:true}}}",
+])
+def test_analyse_option_paths(gpu_ctx, oracle, aopt):
+    print(f"analyse parity: {aopt}", flush=True)
+    frames = synth.clip(320, 192, 2, seed=77, vmax=8.0, nobj=(2, 3), rot_deg=1.0)
+    _check(gpu_ctx, oracle, frames, "{pel:2}", aopt)
+
+
+def test_analyse_batched_padded(gpu_ctx, oracle):
+    print("analyse parity: batch of 4 pairs, padded stride", flush=True)
+    frames = synth.clip(352, 200, 5, seed=91, vmax=6.0, nobj=(2, 2), rot_deg=0.0)
+    _check(gpu_ctx, oracle, frames, "{pel:2}", CFG2, pad=24, batch=True)
