@@ -708,7 +708,8 @@ __global__ void interp_kernel(const Transition* __restrict__ tr) {
         DVec o{};
         o.dx = 0; o.dy = 0; o.score = 0; o.luma = 0; o.done = 0;
         t.proj_key[i] = ~0ull;
-        if (x < t.gw - 1 && y < t.gh - 1) {
+        // top level (no coarser field): every block starts at Vec8::default()
+        if (t.coarse && x < t.gw - 1 && y < t.gh - 1) {
             const int cw = t.cgw - 1, chh = t.cgh - 1;
             DVec p;
             if (x < 2 * cw && y < 2 * chh) {
@@ -731,6 +732,7 @@ __global__ void interp_kernel(const Transition* __restrict__ tr) {
 // part 2: scatter the minimum-score key to each projection target (raster order breaks ties).
 __global__ void project_kernel(const Transition* __restrict__ tr) {
     const Transition t = tr[blockIdx.y];
+    if (!t.coarse) return;
     const int cw = t.cgw - 1, chh = t.cgh - 1;
     const int n = (t.gw - 1) * (t.gh - 1);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -749,6 +751,7 @@ __global__ void project_kernel(const Transition* __restrict__ tr) {
 // part 3: materialise the projected vectors (negated) from the winning keys.
 __global__ void project_finish_kernel(const Transition* __restrict__ tr) {
     const Transition t = tr[blockIdx.y];
+    if (!t.coarse) return;
     const int n = t.gw * t.gh;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const unsigned long long key = t.proj_key[i];

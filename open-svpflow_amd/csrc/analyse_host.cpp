@@ -238,6 +238,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
         seg_off[l] = blob.take(sizeof(int) * 2 * (nchains + 1));
     }
     size_t pack_off = blob.take(sizeof(PackJob) * nchains);
+    size_t init_off = blob.take(sizeof(Transition) * nchains);
     std::vector<uint8_t> host(blob.off, 0);
 
     // ---- device workspace layout (per chain per level buffers)
@@ -336,6 +337,8 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
         return s;
     };
     int nch_step[16] = {}, ntr_step[16] = {}, nsort_step[16] = {}, max_tr_n[16] = {}, max_sort_n[16] = {};
+    Transition* inits = (Transition*)&host[init_off];
+    int ninit = 0, max_init_n = 0;
     for (int step = max_levels - 1; step >= 0; step--) {
         const int l = step;  // aligned at level 0: every clip runs level l at step l
         Chain* chains = (Chain*)&host[chain_off[l]];
@@ -367,6 +370,18 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
             ch.geom = (const LevelGeom*)blob_ptr(c.geom_off) + l;
             ch.global = top ? nullptr : (const int32_t*)(W + cb[ci].glob[l]);
             chains[nc++] = ch;
+            if (top) {
+                // clear the top level's field (the workspace is reused across calls)
+                Transition t{};
+                t.coarse = nullptr;
+                t.next = (DVec*)(W + cb[ci].field[l]);
+                t.proj_key = (unsigned long long*)(W + cb[ci].key[l]);
+                t.proj = (DVec*)(W + cb[ci].proj[l]);
+                t.gw = c.gw[l];
+                t.gh = c.gh[l];
+                inits[ninit++] = t;
+                max_init_n = std::max(max_init_n, c.gw[l] * c.gh[l]);
+            }
             if (!top) {
                 // transition from level l+1 into level l
                 Transition t{};
@@ -463,6 +478,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
     if (d_blob != first_blob) return fail(SVPF_ERR_STATE, "svpf_analyse: descriptor buffer moved");
 
     // ---- launches
+    if (ninit) SVPF_HIP(launch_interp(blob_ptr(init_off), ninit, max_init_n, ctx->stream));
     for (int l = max_levels - 1; l >= 0; l--) {
         if (ntr_step[l]) {
             SVPF_HIP(launch_interp(blob_ptr(tr_off[l]), ntr_step[l], max_tr_n[l], ctx->stream));
