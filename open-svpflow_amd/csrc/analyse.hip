@@ -66,6 +66,57 @@ __device__ __forceinline__ uint32_t had4(const int d[16]) {
     return (uint32_t)max(s, 0);
 }
 
+
+typedef uint32_t __attribute__((aligned(1))) u32u;
+__device__ __forceinline__ uint32_t ld32(const uint8_t* p) { return *reinterpret_cast<const u32u*>(p); }
+
+// Sum of |H4 D H4^T| for one 4x4 tile given four packed rows of source and reference.
+__device__ __forceinline__ uint32_t had4_rows(const uint32_t a[4], const uint32_t b[4]) {
+    int d[16];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) d[r * 4 + c] = (int)((a[r] >> (8 * c)) & 255) - (int)((b[r] >> (8 * c)) & 255);
+    return had4(d);
+}
+
+// Interior block cost: every source and reference byte lies inside its level, so rows
+// are read as 32-bit words (unaligned) with no per-byte checks.  N x M block.
+template <int N, int M>
+__device__ __forceinline__ uint32_t block_fast(const uint8_t* s, int ss, const uint8_t* r, int rs, bool satd) {
+    uint32_t total = 0;
+    if (satd) {
+#pragma unroll
+        for (int ty = 0; ty < M / 4; ty++)
+#pragma unroll
+            for (int tx = 0; tx < N / 4; tx++) {
+                uint32_t a[4], b[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    a[k] = ld32(s + (ty * 4 + k) * ss + tx * 4);
+                    b[k] = ld32(r + (ty * 4 + k) * rs + tx * 4);
+                }
+                total += had4_rows(a, b);
+            }
+        return total >> 1;
+    }
+#pragma unroll
+    for (int y = 0; y < M; y++)
+#pragma unroll
+        for (int x = 0; x < N; x += 4) total = __builtin_amdgcn_sad_u8(ld32(s + y * ss + x), ld32(r + y * rs + x), total);
+    return total;
+}
+
+// Dispatch of block_fast over the block sizes the search uses; false = not handled.
+__device__ __forceinline__ bool block_fast_any(int n, int m, const uint8_t* s, int ss, const uint8_t* r, int rs,
+                                               bool satd, uint32_t& out) {
+    if (n == 8 && m == 8) { out = block_fast<8, 8>(s, ss, r, rs, satd); return true; }
+    if (n == 4 && m == 4) { out = block_fast<4, 4>(s, ss, r, rs, satd); return true; }
+    if (n == 16 && m == 16) { out = block_fast<16, 16>(s, ss, r, rs, satd); return true; }
+    if (n == 32 && m == 32) { out = block_fast<32, 32>(s, ss, r, rs, satd); return true; }
+    return false;
+}
+
 // The block being searched (uniform across the wave).
 struct Blk {
     const uint8_t* sp[3];
@@ -78,7 +129,7 @@ struct Blk {
 };
 
 // satd_edge (analyse_search.rs:2305-2351): all coordinates clamped into the plane.
-__device__ uint32_t satd_edge(const uint8_t* s, int64_t sl, int ss, int srow, int sx0, int sy0, const uint8_t* r,
+__device__ __attribute__((always_inline)) uint32_t satd_edge(const uint8_t* s, int64_t sl, int ss, int srow, int sx0, int sy0, const uint8_t* r,
                               int64_t rl, int rs, int rrow, int rx0, int ry0, int w, int h, int n) {
     uint32_t total = 0;
     const int mw = max(w - 1, 0), mh = max(h - 1, 0);
@@ -102,7 +153,7 @@ __device__ uint32_t satd_edge(const uint8_t* s, int64_t sl, int ss, int srow, in
 // Candidate cost (luma, 4x chroma) by the reference's path selection:
 // out-of-bounds mv -> block_cost_edge_lc (:1996-2167); otherwise
 // block_cost_luma_interior (:1719-1852) + chroma_sad_x4 (:2169-2303).
-__device__ void cand_cost(const Blk& b, int mvx, int mvy, bool chroma, uint32_t& L, uint32_t& C) {
+__device__ __attribute__((noinline)) void cand_cost(const Blk& b, int mvx, int mvy, bool chroma, uint32_t& L, uint32_t& C) {
     const int pel = b.pel;
     const int b0 = -pel * b.px, b1 = -pel * b.py;
     const int b2 = pel * (b.lw - b.px - b.bw) + 1, b3 = pel * (b.lh - b.py - b.bh) + 1;
@@ -173,7 +224,16 @@ __device__ void cand_cost(const Blk& b, int mvx, int mvy, bool chroma, uint32_t&
     const int rrow = b.y_off + sub * b.lh;
     const uint8_t* S0 = b.sp[0];
     const uint8_t* R0 = b.rp[0];
-    if (sq) {  // satd_luma_clamp (:2353-2436): source outside the level reads 0, reference clamped
+    const int frx = ipx + mxp, fry = ipy + myp;
+    bool done_luma = false;
+    if (ipx >= 0 && ipy >= 0 && ipx + b.bw <= b.lw && ipy + b.bh <= b.lh && frx >= 0 && fry >= 0 &&
+        frx + b.bw <= b.lw && fry + b.bh <= b.lh && (sq || (b.bw == b.bh))) {
+        // interior: satd_interior / sad_u8_simd read exactly these rows
+        done_luma = block_fast_any(b.bw, b.bh, S0 + (int64_t)(b.y_off + ipy) * b.ss[0] + ipx, b.ss[0],
+                                   R0 + (int64_t)(rrow + fry) * b.rs[0] + frx, b.rs[0], sq, L);
+    }
+    if (done_luma) {
+    } else if (sq) {  // satd_luma_clamp (:2353-2436): source outside the level reads 0, reference clamped
         uint32_t total = 0;
         const int n = b.bw;
         for (int ty = 0; ty < n / 4; ty++)
@@ -223,6 +283,17 @@ __device__ void cand_cost(const Blk& b, int mvx, int mvy, bool chroma, uint32_t&
         const int rx0 = cpx + mcx, ry0 = cpy + mcy;
         if (cpx + cw <= b.clw && cpy + ch <= b.clh && rx0 >= 0 && ry0 >= 0 && rx0 + cw <= b.clw && ry0 + ch <= b.clh) {
             uint32_t acc = 0;
+            if (cw == ch) {
+                uint32_t vu, vv;
+                const bool fu = block_fast_any(cw, ch, b.sp[1] + (int64_t)(b.c_off + cpy) * b.ss[1] + cpx, b.ss[1],
+                                               b.rp[1] + (int64_t)(cref + ry0) * b.rs[1] + rx0, b.rs[1], b.satd, vu);
+                const bool fv = block_fast_any(cw, ch, b.sp[2] + (int64_t)(b.c_off + cpy) * b.ss[2] + cpx, b.ss[2],
+                                               b.rp[2] + (int64_t)(cref + ry0) * b.rs[2] + rx0, b.rs[2], b.satd, vv);
+                if (fu && fv) {
+                    C = sat_mul(sat_add(vu, vv), 4);
+                    return;
+                }
+            }
             for (int pl = 1; pl < 3; pl++) {
                 const int64_t sbase = (int64_t)(b.c_off + cpy) * b.ss[pl] + cpx;
                 const int64_t rbase = (int64_t)(cref + ry0) * b.rs[pl] + rx0;
@@ -301,7 +372,7 @@ __device__ __forceinline__ void wave_argmin(int64_t& c, int& k) {
 // Updates `best` as the reference's sequence of exact_check calls would and returns
 // the index of the candidate that last improved it, or -1.
 template <class Gen>
-__device__ int check_list(const Blk& b, const Search& s, int n, Gen gen, Best& best) {
+__device__ __attribute__((always_inline)) int check_list(const Blk& b, const Search& s, int n, Gen gen, Best& best) {
     const int lane = threadIdx.x;
     int win = -1;
     for (int base = 0; base < n; base += kWave) {
@@ -355,7 +426,7 @@ __device__ int check_list(const Blk& b, const Search& s, int n, Gen gen, Best& b
 }
 
 // exact_expanding (:829-886): square ring of `radius` around (cx, cy) with `step`.
-__device__ void expanding(const Blk& b, const Search& s, int cx, int cy, int r, int step, Best& best) {
+__device__ __attribute__((always_inline)) void expanding(const Blk& b, const Search& s, int cx, int cy, int r, int step, Best& best) {
     const int m = (2 * r - 1) / step;  // values of i in -r+step, -r+2step, ... < r
     const int n = 4 * m + 4;
     check_list(b, s, n,
@@ -382,7 +453,7 @@ __constant__ int kHex4[16][2] = {{-4, 2}, {-4, 1}, {-4, 0}, {-4, -1}, {-4, -2}, 
                                  {4, 1},  {4, 2},  {2, 3},  {0, 4},   {-2, 3},  {-2, -3}, {0, -4}, {2, -3}};
 
 // exact_hex2 (:941-1033).
-__device__ void hex2(const Blk& b, const Search& s, int radius, Best& best) {
+__device__ __attribute__((always_inline)) void hex2(const Blk& b, const Search& s, int radius, Best& best) {
     int bmx = best.mvx, bmy = best.mvy;
     if (radius > 1) {
         int dir = -2;
@@ -410,7 +481,7 @@ __device__ void hex2(const Blk& b, const Search& s, int radius, Best& best) {
 }
 
 // exact_umh (:1036-1119): cross, multi-scale 16-point hexagon, then hex2.
-__device__ void umh(const Blk& b, const Search& s, int cx, int cy, int radius, Best& best) {
+__device__ __attribute__((always_inline)) void umh(const Blk& b, const Search& s, int cx, int cy, int radius, Best& best) {
     const int nd = radius / 2;                   // odd d in [1, radius)
     const int ns = max(radius / 4, 1);
     const int n = 4 * nd + 16 * ns;
@@ -435,7 +506,7 @@ __device__ void umh(const Blk& b, const Search& s, int cx, int cy, int radius, B
 }
 
 // exact_refine_search (:771-826).
-__device__ void refine(const Blk& b, const Search& s, int type, int radius, Best& best) {
+__device__ __attribute__((always_inline)) void refine(const Blk& b, const Search& s, int type, int radius, Best& best) {
     if (type == 2) {
         hex2(b, s, radius, best);
     } else if (type == 3) {
@@ -447,7 +518,7 @@ __device__ void refine(const Blk& b, const Search& s, int type, int radius, Best
 }
 
 // adapt_lambda (:1555-1564), IEEE double as the reference.
-__device__ int adapt_lambda(int nl, int lsad, uint32_t score) {
+__device__ __attribute__((always_inline)) int adapt_lambda(int nl, int lsad, uint32_t score) {
     const double l = (double)max(lsad, 1);
     const double half = score == 0xFFFFFFFFu ? -1.0 : (double)(score >> 1);
     const double t = __ddiv_rn(l, __dadd_rn(l, half));
@@ -459,7 +530,7 @@ __device__ int adapt_lambda(int nl, int lsad, uint32_t score) {
 }
 
 // Per-lane byte sum helper for block_luma_dc (:1566-1617) and plane activity.
-__device__ uint8_t luma_dc(const Blk& b) {
+__device__ __attribute__((always_inline)) uint8_t luma_dc(const Blk& b) {
     const int lane = threadIdx.x;
     uint32_t sum = 0, cnt = 0;
     // the reference stops a row at the first column past the level, and stops at the first row past it
@@ -479,7 +550,7 @@ __device__ uint8_t luma_dc(const Blk& b) {
 }
 
 // plane_activity (:1517-1553) of one plane region, wave-parallel.
-__device__ uint32_t plane_activity(const uint8_t* p, int stride, int row, int x, int w, int h, bool satd) {
+__device__ __attribute__((always_inline)) uint32_t plane_activity(const uint8_t* p, int stride, int row, int x, int w, int h, bool satd) {
     const int lane = threadIdx.x;
     const uint32_t area = (uint32_t)(w * h);
     uint32_t sum = 0;
