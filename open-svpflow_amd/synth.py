@@ -8,6 +8,9 @@ Content model (integer arithmetic only, so the bytes are reproducible):
   * textured rectangles (own seeds, own velocities) painted in order, optionally
     rotated by a per-frame angle with nearest-neighbour sampling in Q16;
   * chroma: an independent blurred texture at half resolution moved by floor(v/2);
+  * luma is mapped to 16..127 (16 + 7*t/16): the scene detector's luma weight
+    wraps to a tiny value once the two frames' block means sum past 255, so content
+    centred on mid-gray would be classified as a scene change in every frame;
   * scene cuts: every `cut_every` frames the seed advances and everything is redrawn.
 """
 from __future__ import annotations
@@ -143,6 +146,7 @@ def clip(width: int, height: int, frames: int, seed: int, vmax: float = 6.0, nob
         V = _sample_int(scene.ctex[1], coy, cox, height // 2, width // 2).copy()
         for o in scene.objs:
             _paint(Y, U, V, o, lt)
+        Y = (16 + ((Y.astype(np.uint16) * 7) >> 4)).astype(np.uint8)
         out.append((Y, U, V))
     return out
 

@@ -221,3 +221,63 @@ int oracle_rescale(int16_t* dxdy, uint32_t* score, int n, int width, int height,
 }
 
 }  // extern "C"
+
+#include "render.h"
+
+extern "C" {
+
+// SmoothFps create-time options: 0 ok, -3 with the reference's error text.
+int oracle_smooth_opts(const char* opt, size_t len, int64_t fps_num, int64_t fps_den, const int32_t* vdata16,
+                       int32_t nframes, int32_t* out_frames, int32_t* algo, int32_t* scene_mode, char* err,
+                       size_t errlen) {
+    Value v;
+    bool have = parse_opt(opt, len, &v);
+    VectorData vd;
+    if (!parse_vdata(vdata16, &vd)) return -1;
+    SmoothOptions o;
+    std::string e;
+    if (!smooth_options(have ? &v : nullptr, fps_num, fps_den, vd, &o, &e)) {
+        put_err(err, errlen, e);
+        return -3;
+    }
+    *out_frames = o.timing(fps_num, fps_den).scale_frame_count(nframes);
+    *algo = (int32_t)o.algo;
+    *scene_mode = (int32_t)o.scene_mode;
+    return 0;
+}
+
+// Render output frame k of a clip held in host memory (planes packed at stride = width).
+int oracle_smooth_render(const char* opt, size_t len, int width, int height, int64_t fps_num, int64_t fps_den,
+                         int nframes, int64_t sdata, const int32_t* vdata16, const uint8_t* const* src_y,
+                         const uint8_t* const* src_u, const uint8_t* const* src_v, const uint8_t* const* sup_y,
+                         const uint8_t* const* sup_u, const uint8_t* const* sup_v, int super_rows,
+                         const uint8_t* const* vec, size_t vec_len, int k, uint8_t* dy, uint8_t* du, uint8_t* dv,
+                         int32_t* algo_out) {
+    Value v;
+    bool have = parse_opt(opt, len, &v);
+    SmoothClip c;
+    c.width = width;
+    c.height = height;
+    c.num_frames = nframes;
+    c.fps_num = fps_num;
+    c.fps_den = fps_den;
+    c.sdata = sdata;
+    if (!parse_vdata(vdata16, &c.vd)) return -1;
+    std::string e;
+    if (!smooth_options(have ? &v : nullptr, fps_num, fps_den, c.vd, &c.opt, &e)) return -3;
+    for (int i = 0; i < nframes; i++) {
+        c.src[0].push_back(src_y[i]); c.src[1].push_back(src_u[i]); c.src[2].push_back(src_v[i]);
+        c.sup[0].push_back(sup_y ? sup_y[i] : nullptr); c.sup[1].push_back(sup_u ? sup_u[i] : nullptr);
+        c.sup[2].push_back(sup_v ? sup_v[i] : nullptr);
+        c.vec.push_back(vec[i]);
+    }
+    c.super_rows = super_rows;
+    c.vec_len = vec_len;
+    uint8_t* dst[3] = {dy, du, dv};
+    int algo = 0;
+    int r = smooth_render_frame(c, k, dst, &algo);
+    *algo_out = algo;
+    return r;
+}
+
+}  // extern "C"

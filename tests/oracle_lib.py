@@ -166,3 +166,64 @@ def decode_payload(payload: np.ndarray):
                               (d[:, 1] >> 24).astype(np.int64)], 1))
         off += 4 + 8 * count
     return hdr, regs
+
+_pp = C.POINTER(C.c_void_p)
+L.oracle_smooth_opts.argtypes = [C.c_char_p, _sz, C.c_int64, C.c_int64, C.POINTER(C.c_int32), C.c_int32,
+                                 C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_char_p, _sz]
+L.oracle_smooth_render.argtypes = [C.c_char_p, _sz, _i, _i, C.c_int64, C.c_int64, _i, C.c_int64,
+                                   C.POINTER(C.c_int32)] + [_pp] * 6 + [_i, _pp, _sz, _i, u8p, u8p, u8p,
+                                                                         C.POINTER(C.c_int32)]
+
+
+def smooth_opts(opt, fps_num, fps_den, vdata, nframes):
+    b = (opt or "").encode()
+    vd = (C.c_int32 * 16)(*vdata)
+    of, algo, sm = C.c_int32(), C.c_int32(), C.c_int32()
+    err = C.create_string_buffer(256)
+    rc = L.oracle_smooth_opts(b, len(b), fps_num, fps_den, vd, nframes, C.byref(of), C.byref(algo), C.byref(sm),
+                              err, 256)
+    if rc == -3:
+        return None, err.value.decode()
+    assert rc == 0
+    return dict(out_frames=of.value, algo=algo.value, scene_mode=sm.value), None
+
+
+class SmoothClip:
+    """Host-side clip state for the oracle renderer: sources, super frames, vector payloads."""
+
+    def __init__(self, opt, frames, supers, payloads, sdata, vdata, fps=(24, 1)):
+        self.opt = (opt or "").encode()
+        self.h, self.w = frames[0][0].shape
+        self.n = len(frames)
+        self.fps = fps
+        self.sdata = sdata
+        self.vdata = (C.c_int32 * 16)(*vdata)
+        self.keep = []
+
+        def ptrs(arrs):
+            arrs = [np.ascontiguousarray(a) for a in arrs]
+            self.keep.extend(arrs)
+            return (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+        self.src = [ptrs([f[p] for f in frames]) for p in range(3)]
+        if supers is not None:
+            self.sup = [ptrs([s[p][:, :(self.w if p == 0 else self.w // 2)] for s in supers]) for p in range(3)]
+            self.super_rows = supers[0][0].shape[0]
+        else:
+            self.sup = [None, None, None]
+            self.super_rows = 0
+        self.vec = ptrs(payloads)
+        self.vec_len = len(payloads[0])
+
+    def render(self, k):
+        Y = np.zeros((self.h, self.w), np.uint8)
+        U = np.zeros((self.h // 2, self.w // 2), np.uint8)
+        V = np.zeros((self.h // 2, self.w // 2), np.uint8)
+        algo = C.c_int32()
+        sup = [s if s is not None else C.cast(None, _pp) for s in self.sup]
+        rc = L.oracle_smooth_render(self.opt, len(self.opt), self.w, self.h, self.fps[0], self.fps[1], self.n,
+                                    self.sdata, self.vdata, self.src[0], self.src[1], self.src[2], sup[0], sup[1],
+                                    sup[2], self.super_rows, self.vec, self.vec_len, k, _p(Y), _p(U), _p(V),
+                                    C.byref(algo))
+        assert rc >= 0, rc
+        return (Y, U, V), rc, algo.value
