@@ -133,6 +133,36 @@ int svpf_analyse_batch(svpf_ctx* ctx, const svpf_analyse_params* p, int32_t n, c
 int svpf_analyse_multi(svpf_ctx* ctx, int32_t n, const svpf_analyse_params* const* params, const svpf_frame* d_cur,
                        const svpf_frame* d_ref, uint8_t* const* d_payload);
 
+/* ------------------------------------------------------------------ SmoothFps */
+typedef struct svpf_smooth svpf_smooth;
+
+/* Creates a render session for one clip: the SmoothFps option string, the source
+ * frame rate and length, the Super clip's `sdata` and the vector clip's 16-word
+ * `vdata` header.  Replaces svpflow2 create_filter + Options::from_value
+ * (crates/svpflow2/src/filter.rs:60-130, options.rs:150-260): the same option
+ * keys and defaults, and on rejection SVPF_ERR_OPT with the reference's error text
+ * in `err` ("SVSmoothFps: incorrect 'algo' value", ...).  *out_frames receives
+ * the output clip length (frame_math.rs:128-135).  Renders YUV420P8 with the
+ * reference's integer CPU renderer semantics (renderer.rs). */
+int svpf_smooth_create(svpf_ctx* ctx, const char* opt, size_t opt_len, int32_t width, int32_t height, int64_t fps_num,
+                       int64_t fps_den, int32_t nframes, int64_t sdata, const int32_t* vdata16, svpf_smooth** out,
+                       int32_t* out_frames, char* err, size_t err_len);
+void svpf_smooth_destroy(svpf_smooth* s);
+/* Binds source frames first..first+count-1: the source planes, their Super frames
+ * (d_super may be NULL when the vectors are pel 1) and the device vector payloads
+ * written by svpf_analyse_* (the frame's getFrame inputs, core.rs:962-1051).  All
+ * source frames share strides, as do all Super frames.  Binding marks the frame's
+ * decoded vectors stale. */
+int svpf_smooth_set_sources(svpf_smooth* s, int32_t first, int32_t count, const svpf_frame* d_src,
+                            const svpf_frame* d_super, uint8_t* const* d_vectors);
+/* Renders output frames k0..k0+count-1 into d_out[0..count-1] in one batch of
+ * launches on the context's stream (get_frame -> try_render -> render_into_output,
+ * core.rs:752-1770).  If h_decision is not NULL it receives, per frame, the pair
+ * (kind, algo): kind 0 = copy of source n, 1 = copy of n+1, 2 = interpolated with
+ * `algo`; the call then synchronizes. */
+int svpf_smooth_render(svpf_smooth* s, int32_t k0, int32_t count, const svpf_frame* d_out, int32_t* h_decision);
+int32_t svpf_smooth_output_frames(const svpf_smooth* s);
+
 #ifdef __cplusplus
 }
 #endif

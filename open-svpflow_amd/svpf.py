@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -92,9 +93,12 @@ class Context:
         check(lib().svpf_create(device, C.byref(h)))
         self.h = h
         self._allocs: list[int] = []
+        self._sessions = weakref.WeakSet()  # render sessions to end before the context
 
     def close(self) -> None:
         if self.h:
+            for s in list(self._sessions):
+                s.close()
             for p in self._allocs:
                 lib().svpf_device_free(self.h, p)
             self._allocs.clear()
@@ -273,3 +277,74 @@ def analyse_multi(ctx: Context, params, curs, refs, d_payloads) -> None:
     R = (Frame * n)(*[f.desc() for f in refs])
     P = (C.c_void_p * n)(*d_payloads)
     check(lib().svpf_analyse_multi(ctx.h, n, PP, Cu, R, P))
+
+
+# ---------------------------------------------------------------- SmoothFps
+
+def _bind_smooth(L):
+    L.svpf_smooth_create.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t, C.c_int32, C.c_int32, C.c_int64,
+                                     C.c_int64, C.c_int32, C.c_int64, C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_void_p), C.POINTER(C.c_int32), C.c_char_p, C.c_size_t]
+    L.svpf_smooth_destroy.argtypes = [C.c_void_p]
+    L.svpf_smooth_destroy.restype = None
+    L.svpf_smooth_set_sources.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(Frame), C.POINTER(Frame),
+                                          C.POINTER(C.c_void_p)]
+    L.svpf_smooth_render.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(Frame), C.POINTER(C.c_int32)]
+    L.svpf_smooth_output_frames.argtypes = [C.c_void_p]
+    L.svpf_smooth_output_frames.restype = C.c_int32
+
+
+_BINDERS.append(_bind_smooth)
+
+
+class Smooth:
+    """svpflow2.SmoothFps render session for one clip (YUV420P8).
+
+    `opt` is the SmoothFps option string, `sdata` the Super clip's handle and
+    `vdata` the 16-word vector header returned by analyse_params.
+    """
+
+    def __init__(self, ctx: Context, opt, width: int, height: int, fps_num: int, fps_den: int, nframes: int,
+                 sdata: int, vdata):
+        L = lib()
+        self.ctx = ctx
+        h = C.c_void_p()
+        nout = C.c_int32()
+        err = C.create_string_buffer(256)
+        vd = (C.c_int32 * 16)(*vdata)
+        b = opt.encode() if isinstance(opt, str) else (opt or b"")
+        rc = L.svpf_smooth_create(ctx.h, b, len(b), width, height, fps_num, fps_den, nframes, sdata, vd,
+                                  C.byref(h), C.byref(nout), err, 256)
+        if rc == -3:
+            raise SvpfError(err.value.decode())
+        check(rc)
+        self.h = h
+        self.output_frames = nout.value
+        ctx._sessions.add(self)
+
+    def close(self) -> None:
+        if self.h:
+            lib().svpf_smooth_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_sources(self, first: int, srcs, supers, d_vectors) -> None:
+        n = len(srcs)
+        S = (Frame * n)(*[f.desc() for f in srcs])
+        U = (Frame * n)(*[f.desc() for f in supers]) if supers is not None else None
+        V = (C.c_void_p * n)(*d_vectors)
+        check(lib().svpf_smooth_set_sources(self.h, first, n, S, U, V))
+
+    def render(self, k0: int, outs, decisions: bool = False):
+        n = len(outs)
+        O = (Frame * n)(*[f.desc() for f in outs])
+        D = (C.c_int32 * (2 * n))() if decisions else None
+        check(lib().svpf_smooth_render(self.h, k0, n, O, D))
+        if decisions:
+            return [(D[2 * i], D[2 * i + 1]) for i in range(n)]
+        return None
