@@ -58,6 +58,14 @@ int svpf_memset_d(svpf_ctx* ctx, void* d_dst, int value, size_t bytes);
 /* Event timing on the context's stream: start, then stop returns elapsed ms. */
 int svpf_timer_start(svpf_ctx* ctx);
 int svpf_timer_stop(svpf_ctx* ctx, float* ms);
+/* Per-kernel timing for benchmarks (no counterpart in the reference): when enabled,
+ * HIP events are recorded on the context's stream around each launch of "super"
+ * (level 0 + pyramid), "search" (one Analyse level's block search), "analyse_aux"
+ * (reserved), "smooth_prep" (vector prep, plans and masks) and "render".  Enabling
+ * (or disabling) synchronizes and clears the records; read synchronizes and returns
+ * the summed milliseconds and the number of timed launches of one kernel. */
+int svpf_profile_enable(svpf_ctx* ctx, int on);
+int svpf_profile_read(svpf_ctx* ctx, const char* kernel, double* total_ms, int32_t* launches);
 
 /* ------------------------------------------------------------------ Super */
 /* Replaces SuperOpts (crates/svpflow1/src/super_opts.rs:3-95). */

@@ -495,7 +495,11 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
                                     segp + nchains + 1, ctx->stream));
             SVPF_HIP(launch_child_order(blob_ptr(oj_off[l]), nsort_step[l], ctx->stream));
         }
-        if (nch_step[l]) SVPF_HIP(launch_search_level((const Chain*)blob_ptr(chain_off[l]), nch_step[l], ctx->stream));
+        if (nch_step[l]) {
+            const int ph = prof_begin(ctx, kProfSearch);
+            SVPF_HIP(launch_search_level((const Chain*)blob_ptr(chain_off[l]), nch_step[l], ctx->stream));
+            prof_end(ctx, ph);
+        }
     }
     if (denom > 0) {
         SVPF_HIP(launch_pack(blob_ptr(pack_off), npack, max_count, denom, ctx->stream));

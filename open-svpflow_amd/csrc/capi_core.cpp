@@ -16,6 +16,39 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
+int prof_begin(svpf_ctx* ctx, int kernel) {
+    if (!ctx->prof) return -1;
+    hipEvent_t ev[2];
+    for (auto& e : ev) {
+        if (!ctx->prof_pool.empty()) {
+            e = ctx->prof_pool.back();
+            ctx->prof_pool.pop_back();
+        } else if (hipEventCreate(&e) != hipSuccess) {
+            return -1;
+        }
+    }
+    if (hipEventRecord(ev[0], ctx->stream) != hipSuccess) {
+        ctx->prof_pool.push_back(ev[0]);
+        ctx->prof_pool.push_back(ev[1]);
+        return -1;
+    }
+    ctx->prof_marks.push_back(svpf_ctx::ProfMark{kernel, ev[0], ev[1]});
+    return (int)ctx->prof_marks.size() - 1;
+}
+
+void prof_end(svpf_ctx* ctx, int handle) {
+    if (handle < 0 || handle >= (int)ctx->prof_marks.size()) return;
+    (void)hipEventRecord(ctx->prof_marks[(size_t)handle].b, ctx->stream);
+}
+
+static void prof_clear(svpf_ctx* ctx) {
+    for (auto& m : ctx->prof_marks) {
+        ctx->prof_pool.push_back(m.a);
+        ctx->prof_pool.push_back(m.b);
+    }
+    ctx->prof_marks.clear();
+}
+
 int reduce_dim(int dim, int n) {  // super_opts.rs:70-75
     for (int i = 0; i < n; i++) dim = 2 * (dim / 4);
     return dim;
@@ -131,6 +164,8 @@ void svpf_destroy(svpf_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->d_desc) (void)hipFree(ctx->d_desc);
     if (ctx->h_desc) (void)hipHostFree(ctx->h_desc);
+    prof_clear(ctx);
+    for (auto e : ctx->prof_pool) (void)hipEventDestroy(e);
     (void)hipEventDestroy(ctx->desc_done);
     (void)hipEventDestroy(ctx->t0);
     (void)hipEventDestroy(ctx->t1);
@@ -269,13 +304,45 @@ int svpf_super_build_batch(svpf_ctx* ctx, const svpf_super_params* p, int32_t n,
     if (rc) return rc;
     const Planes* ds = (const Planes*)d_desc;
     const Planes* dd = ds + n;
+    const int ph = prof_begin(ctx, kProfSuper);
     SVPF_HIP(launch_super_level0(g, ds, dd, n, max_stride, ctx->stream));
     for (int lv = 0; lv + 1 < p->levels; lv++) SVPF_HIP(launch_super_reduce(g, dd, n, lv, ctx->stream));
+    prof_end(ctx, ph);
     return SVPF_OK;
 }
 
 int svpf_super_build(svpf_ctx* ctx, const svpf_super_params* p, const svpf_frame* d_src, const svpf_frame* d_dst) {
     return svpf_super_build_batch(ctx, p, 1, d_src, d_dst);
+}
+
+int svpf_profile_enable(svpf_ctx* ctx, int on) {
+    if (!ctx) return fail(SVPF_ERR_ARG, "svpf_profile_enable: null context");
+    SVPF_HIP(hipStreamSynchronize(ctx->stream));
+    prof_clear(ctx);
+    ctx->prof = on != 0;
+    return SVPF_OK;
+}
+
+int svpf_profile_read(svpf_ctx* ctx, const char* kernel, double* total_ms, int32_t* launches) {
+    static const char* const kNames[kProfCount] = {"super", "search", "analyse_aux", "smooth_prep", "render"};
+    if (!ctx || !kernel || !total_ms || !launches) return fail(SVPF_ERR_ARG, "svpf_profile_read: null argument");
+    int id = -1;
+    for (int k = 0; k < kProfCount; k++)
+        if (std::strcmp(kernel, kNames[k]) == 0) id = k;
+    if (id < 0) return fail(SVPF_ERR_ARG, std::string("svpf_profile_read: unknown kernel ") + kernel);
+    SVPF_HIP(hipStreamSynchronize(ctx->stream));
+    double sum = 0;
+    int n = 0;
+    for (const auto& m : ctx->prof_marks) {
+        if (m.kernel != id) continue;
+        float ms = 0;
+        SVPF_HIP(hipEventElapsedTime(&ms, m.a, m.b));
+        sum += ms;
+        n++;
+    }
+    *total_ms = sum;
+    *launches = n;
+    return SVPF_OK;
 }
 
 }  // extern "C"

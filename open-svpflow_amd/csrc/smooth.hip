@@ -512,8 +512,8 @@ hipError_t launch_smooth_prep(const PrepJob* d_jobs, int njobs, const SmoothGeom
     return hipGetLastError();
 }
 
-hipError_t launch_smooth_frames(const FrameJob* d_jobs, FramePlan* d_plans, int njobs, const SmoothGeom* d_geom, int width,
-                                int height, int count, int mw, int mh, bool coverage, hipStream_t s) {
+hipError_t launch_smooth_masks(const FrameJob* d_jobs, FramePlan* d_plans, int njobs, const SmoothGeom* d_geom, int count,
+                               int mw, int mh, bool coverage, hipStream_t s) {
     hipLaunchKernelGGL(plan_kernel, dim3((njobs + 63) / 64), dim3(64), 0, s, d_jobs, d_plans, njobs, d_geom);
     const int acc = (mw + 2) * (mh + 2), mc = mw * mh;
     if (coverage) {
@@ -521,6 +521,11 @@ hipError_t launch_smooth_frames(const FrameJob* d_jobs, FramePlan* d_plans, int 
         hipLaunchKernelGGL(cov_splat_kernel, dim3((2 * count + 255) / 256, njobs), dim3(256), 0, s, d_jobs, d_plans, d_geom);
     }
     hipLaunchKernelGGL(cov_finish_kernel, dim3((mc + 255) / 256, njobs), dim3(256), 0, s, d_jobs, d_plans, d_geom);
+    return hipGetLastError();
+}
+
+hipError_t launch_smooth_render(const FrameJob* d_jobs, const FramePlan* d_plans, int njobs, const SmoothGeom* d_geom,
+                                int width, int height, hipStream_t s) {
     dim3 grid((width + 255) / 256, height, 3 * njobs);
     hipLaunchKernelGGL(render_kernel, grid, dim3(256), 0, s, d_jobs, d_plans, d_geom);
     return hipGetLastError();

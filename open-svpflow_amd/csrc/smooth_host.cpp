@@ -13,8 +13,10 @@ namespace svpf {
 
 hipError_t launch_smooth_prep(const PrepJob* d_jobs, int njobs, const SmoothGeom* d_geom, int count, int mc,
                               hipStream_t s);
-hipError_t launch_smooth_frames(const FrameJob* d_jobs, FramePlan* d_plans, int njobs, const SmoothGeom* d_geom, int width,
-                                int height, int count, int mw, int mh, bool coverage, hipStream_t s);
+hipError_t launch_smooth_masks(const FrameJob* d_jobs, FramePlan* d_plans, int njobs, const SmoothGeom* d_geom, int count,
+                               int mw, int mh, bool coverage, hipStream_t s);
+hipError_t launch_smooth_render(const FrameJob* d_jobs, const FramePlan* d_plans, int njobs, const SmoothGeom* d_geom,
+                                int width, int height, hipStream_t s);
 
 namespace {
 
@@ -241,7 +243,9 @@ int ensure_prep(svpf_smooth* s, const std::vector<int>& frames) {
     void* d = nullptr;
     int rc = svpf_upload_desc(s->ctx, jobs.data(), jobs.size() * sizeof(PrepJob), &d);
     if (rc) return rc;
+    const int ph = prof_begin(s->ctx, kProfPrep);
     SVPF_HIP(launch_smooth_prep((const PrepJob*)d, (int)jobs.size(), s->d_geom, s->count, s->mc, s->ctx->stream));
+    prof_end(s->ctx, ph);
     return SVPF_OK;
 }
 
@@ -557,8 +561,13 @@ int svpf_smooth_render(svpf_smooth* s, int32_t k0, int32_t count, const svpf_fra
     rc = svpf_upload_desc(s->ctx, blob.data(), blob.size(), &d_up);
     if (rc) return rc;
     FramePlan* d_plans = (FramePlan*)((uint8_t*)d_up + (size_t)count * sizeof(FrameJob));
-    SVPF_HIP(launch_smooth_frames((const FrameJob*)d_up, d_plans, count, s->d_geom, s->width, s->height, s->count,
-                                  s->mw, s->mh, coverage, s->ctx->stream));
+    int ph = prof_begin(s->ctx, kProfPrep);
+    SVPF_HIP(launch_smooth_masks((const FrameJob*)d_up, d_plans, count, s->d_geom, s->count, s->mw, s->mh, coverage,
+                                 s->ctx->stream));
+    prof_end(s->ctx, ph);
+    ph = prof_begin(s->ctx, kProfRender);
+    SVPF_HIP(launch_smooth_render((const FrameJob*)d_up, d_plans, count, s->d_geom, s->width, s->height, s->ctx->stream));
+    prof_end(s->ctx, ph);
     if (h_decision) {  // (decision, algo) per frame; synchronizes
         SVPF_HIP(hipMemcpyAsync(plans.data(), d_plans, (size_t)count * sizeof(FramePlan), hipMemcpyDeviceToHost,
                                 s->ctx->stream));

@@ -24,6 +24,14 @@ struct svpf_ctx {
     size_t work_cap = 0;
     void* d_sort_tmp = nullptr;
     size_t sort_tmp_cap = 0;
+    // Optional kernel timing: HIP event pairs recorded around launches on `stream`.
+    struct ProfMark {
+        int kernel;
+        hipEvent_t a, b;
+    };
+    bool prof = false;
+    std::vector<ProfMark> prof_marks;
+    std::vector<hipEvent_t> prof_pool;
 };
 
 namespace svpf {
@@ -53,6 +61,13 @@ SuperGeom make_super_geom(const svpf_super_params& p);
 // Error reporting for the C-ABI: thread-local last error string.
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
+
+// Kernel timing (svpf_profile_*): kernels that are timed one launch at a time.
+enum ProfKernel { kProfSuper = 0, kProfSearch, kProfAnalyseAux, kProfPrep, kProfRender, kProfCount };
+// Records the start event of one launch of `kernel` when profiling is on; returns a
+// handle for prof_end (or -1).
+int prof_begin(svpf_ctx* ctx, int kernel);
+void prof_end(svpf_ctx* ctx, int handle);
 
 #define SVPF_HIP(call)                                                                 \
     do {                                                                               \

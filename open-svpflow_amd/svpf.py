@@ -66,6 +66,8 @@ def lib() -> C.CDLL:
         L.svpf_super_build.argtypes = [C.c_void_p, C.POINTER(SuperParams), C.POINTER(Frame), C.POINTER(Frame)]
         L.svpf_super_build_batch.argtypes = [C.c_void_p, C.POINTER(SuperParams), C.c_int32, C.POINTER(Frame),
                                              C.POINTER(Frame)]
+        L.svpf_profile_enable.argtypes = [C.c_void_p, C.c_int]
+        L.svpf_profile_read.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
         for bind in _BINDERS:
             bind(L)
         _lib = L
@@ -151,6 +153,15 @@ class Context:
         ms = C.c_float()
         check(lib().svpf_timer_stop(self.h, C.byref(ms)))
         return ms.value
+
+    def profile(self, on: bool) -> None:
+        check(lib().svpf_profile_enable(self.h, 1 if on else 0))
+
+    def profile_read(self, kernel: str):
+        """(total ms, launches) of one timed kernel since profiling was enabled."""
+        ms, n = C.c_double(), C.c_int32()
+        check(lib().svpf_profile_read(self.h, kernel.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
 
 @dataclass
