@@ -17,6 +17,7 @@
 //     batch of pairs never returns to the host between levels.
 #include <hipcub/hipcub.hpp>
 
+#include "analyse_cost.h"
 #include "analyse_dev.h"
 #include "svpf_internal.h"
 
@@ -24,61 +25,7 @@ namespace svpf {
 namespace {
 
 constexpr int kWave = 64;
-
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
-__device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
-    uint32_t r = a + b;
-    return r < a ? 0xFFFFFFFFu : r;
-}
-__device__ __forceinline__ uint32_t sat_mul(uint32_t a, uint32_t m) {
-    uint64_t r = (uint64_t)a * m;
-    return r > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)r;
-}
-__device__ __forceinline__ int sat_mul_i32(int a, int b) {
-    const long long r = (long long)a * b;
-    return (int)(r > INT32_MAX ? INT32_MAX : (r < INT32_MIN ? INT32_MIN : r));
-}
-__device__ __forceinline__ int sat_add_i32(int a, int b) {
-    const long long r = (long long)a + b;
-    return (int)(r > INT32_MAX ? INT32_MAX : (r < INT32_MIN ? INT32_MIN : r));
-}
-__device__ __forceinline__ bool pow2_4_32(int v) { return v == 4 || v == 8 || v == 16 || v == 32; }
-// A byte of a plane with the reference's slice semantics (`get(i).unwrap_or(fb)`).
-__device__ __forceinline__ int rd(const uint8_t* p, int64_t len, int64_t i, int fb) {
-    return (i >= 0 && i < len) ? (int)p[i] : fb;
-}
-
-// Sum of |H4 D H4^T| over one 4x4 tile (analyse_search.rs:2681-2702).
-__device__ __forceinline__ uint32_t had4(const int d[16]) {
-    int t[16];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        int s0 = d[i * 4 + 0] + d[i * 4 + 1], s1 = d[i * 4 + 0] - d[i * 4 + 1];
-        int s2 = d[i * 4 + 2] + d[i * 4 + 3], s3 = d[i * 4 + 2] - d[i * 4 + 3];
-        t[i * 4 + 0] = s0 + s2; t[i * 4 + 1] = s1 + s3; t[i * 4 + 2] = s0 - s2; t[i * 4 + 3] = s1 - s3;
-    }
-    int s = 0;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        int s0 = t[j] + t[4 + j], s1 = t[j] - t[4 + j], s2 = t[8 + j] + t[12 + j], s3 = t[8 + j] - t[12 + j];
-        s += abs(s0 + s2) + abs(s1 + s3) + abs(s0 - s2) + abs(s1 - s3);
-    }
-    return (uint32_t)max(s, 0);
-}
-
-
-typedef uint32_t __attribute__((aligned(1))) u32u;
-__device__ __forceinline__ uint32_t ld32(const uint8_t* p) { return *reinterpret_cast<const u32u*>(p); }
-
-// Sum of |H4 D H4^T| for one 4x4 tile given four packed rows of source and reference.
-__device__ __forceinline__ uint32_t had4_rows(const uint32_t a[4], const uint32_t b[4]) {
-    int d[16];
-#pragma unroll
-    for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) d[r * 4 + c] = (int)((a[r] >> (8 * c)) & 255) - (int)((b[r] >> (8 * c)) & 255);
-    return had4(d);
-}
+using namespace cost;
 
 // Interior block cost: every source and reference byte lies inside its level, so rows
 // are read as 32-bit words (unaligned) with no per-byte checks.  N x M block.
@@ -117,227 +64,119 @@ __device__ __forceinline__ bool block_fast_any(int n, int m, const uint8_t* s, i
     return false;
 }
 
-// The block being searched (uniform across the wave).
-struct Blk {
-    const uint8_t* sp[3];
-    const uint8_t* rp[3];
-    int ss[3], rs[3];
-    int64_t sl[3], rl[3];
-    int level, lw, lh, y_off, c_off, clw, clh, pel;
-    int px, py, bw, bh;
-    bool satd;
-};
+// ------------------------------------------------------------------ cooperative cost
 
-// satd_edge (analyse_search.rs:2305-2351): all coordinates clamped into the plane.
-__device__ __attribute__((always_inline)) uint32_t satd_edge(const uint8_t* s, int64_t sl, int ss, int srow, int sx0, int sy0, const uint8_t* r,
-                              int64_t rl, int rs, int rrow, int rx0, int ry0, int w, int h, int n) {
-    uint32_t total = 0;
-    const int mw = max(w - 1, 0), mh = max(h - 1, 0);
-    for (int ty = 0; ty < n / 4; ty++)
-        for (int tx = 0; tx < n / 4; tx++) {
-            int d[16];
-#pragma unroll
-            for (int rr = 0; rr < 4; rr++) {
-                int sy = clampi(sy0 + ty * 4 + rr, 0, mh), ry = clampi(ry0 + ty * 4 + rr, 0, mh);
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    int sx = clampi(sx0 + tx * 4 + c, 0, mw), rx = clampi(rx0 + tx * 4 + c, 0, mw);
-                    d[rr * 4 + c] = rd(s, sl, (int64_t)(srow + sy) * ss + sx, 0) - rd(r, rl, (int64_t)(rrow + ry) * rs + rx, 0);
-                }
-            }
-            total = sat_add(total, had4(d));
-        }
-    return total >> 1;
+constexpr int kGroup = 8;                 // lanes per candidate
+constexpr int kPerPass = kWave / kGroup;  // candidates per pass
+
+// Wave-uniform reads of one lane's value (scalar results keep the search state in SGPRs).
+__device__ __forceinline__ int rl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ uint32_t rlu(uint32_t v, int lane) { return (uint32_t)__builtin_amdgcn_readlane((int)v, lane); }
+__device__ __forceinline__ int64_t rl64(int64_t v, int lane) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), lane);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-// Candidate cost (luma, 4x chroma) by the reference's path selection:
-// out-of-bounds mv -> block_cost_edge_lc (:1996-2167); otherwise
-// block_cost_luma_interior (:1719-1852) + chroma_sad_x4 (:2169-2303).
-__device__ __attribute__((noinline)) void cand_cost(const Blk& b, int mvx, int mvy, bool chroma, uint32_t& L, uint32_t& C) {
+__device__ __forceinline__ uint32_t xor_sum8(uint32_t v) {
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    v += __shfl_xor(v, 4);
+    return v;
+}
+
+// One candidate's cost computed by the 8 lanes of a group (part = lane & 7); every
+// lane of the group receives (L, C).  Blocks whose luma and chroma lie inside their
+// levels (the common case) split into units -- 4x4 tiles for SATD, rows for SAD --
+// spread over the group; any other path is computed whole by part 0 (cand_cost_full).
+__device__ __forceinline__ void group_cost(const Blk& b, int mvx, int mvy, bool chroma, int part, uint32_t& Lout,
+                                           uint32_t& Cout) {
     const int pel = b.pel;
     const int b0 = -pel * b.px, b1 = -pel * b.py;
     const int b2 = pel * (b.lw - b.px - b.bw) + 1, b3 = pel * (b.lh - b.py - b.bh) + 1;
-    const int ox = mvx < b0 ? mvx - b0 : ((b2 > b0 && mvx >= b2) ? mvx + 1 - b2 : 0);
-    const int oy = mvy < b1 ? mvy - b1 : ((b3 > b1 && mvy >= b3) ? mvy + 1 - b3 : 0);
+    const bool edge = mvx < b0 || (b2 > b0 && mvx >= b2) || mvy < b1 || (b3 > b1 && mvy >= b3);
     const bool sq = b.satd && pow2_4_32(b.bw) && pow2_4_32(b.bh) && b.bw == b.bh;
     const bool subpel = b.level == 0 && pel >= 2;
-    if (ox != 0 || oy != 0) {
-        // ---- edge path (edge_shift_origins :1638-1676 + block_cost_edge_lc)
-        const int cx = clampi(b.px - ox, 0, max(b.lw - b.bw, 0));
-        const int cy = clampi(b.py - oy, 0, max(b.lh - b.bh, 0));
-        int rx0, ry0;
-        if (pel == 1) { rx0 = cx + mvx; ry0 = cy + mvy; }
-        else if (pel == 4) { rx0 = (4 * cx + mvx) >> 2; ry0 = (4 * cy + mvy) >> 2; }
-        else { rx0 = (2 * cx + mvx) >> 1; ry0 = (2 * cy + mvy) >> 1; }
-        const int sub = subpel ? (mvy & 1) * pel + (mvx & 1) : 0;
-        const int rrow = b.y_off + sub * b.lh;
-        if (sq) {
-            L = satd_edge(b.sp[0], b.sl[0], b.ss[0], b.y_off, cx, cy, b.rp[0], b.rl[0], b.rs[0], rrow, rx0, ry0, b.lw,
-                          b.lh, b.bw);
-        } else {
-            uint32_t sad = 0;
-            const int mw = max(b.lw - 1, 0), mh = max(b.lh - 1, 0);
-            for (int row = 0; row < b.bh; row++) {
-                const int sy = clampi(cy + row, 0, mh), ry = clampi(ry0 + row, 0, mh);
-                for (int col = 0; col < b.bw; col++) {
-                    const int sx = clampi(cx + col, 0, mw), rx = clampi(rx0 + col, 0, mw);
-                    sad += (uint32_t)abs(rd(b.sp[0], b.sl[0], (int64_t)(b.y_off + sy) * b.ss[0] + sx, 0) -
-                                         rd(b.rp[0], b.rl[0], (int64_t)(rrow + ry) * b.rs[0] + rx, 0));
-                }
-            }
-            L = sad;
-        }
-        C = 0;
-        if (chroma) {
-            const int cw = b.bw / 2, ch = b.bh / 2;
-            const int ccx = (b.px >> 1) - ((b.px - cx) >> 1), ccy = (b.py >> 1) - ((b.py - cy) >> 1);
-            const int crx0 = ((mvx >> 1) + 2 * ccx) >> 1, cry0 = ((mvy >> 1) + 2 * ccy) >> 1;
-            const int cref = b.c_off + sub * b.clh;
-            uint32_t uv;
-            if (b.satd && cw == ch && (cw == 4 || cw == 8 || cw == 16)) {
-                uv = sat_add(satd_edge(b.sp[1], b.sl[1], b.ss[1], b.c_off, ccx, ccy, b.rp[1], b.rl[1], b.rs[1], cref, crx0,
-                                       cry0, b.clw, b.clh, cw),
-                             satd_edge(b.sp[2], b.sl[2], b.ss[2], b.c_off, ccx, ccy, b.rp[2], b.rl[2], b.rs[2], cref, crx0,
-                                       cry0, b.clw, b.clh, cw));
-            } else {
-                uv = 0;
-                const int mw = max(b.clw - 1, 0), mh = max(b.clh - 1, 0);
-                for (int row = 0; row < ch; row++) {
-                    const int sy = clampi(ccy + row, 0, mh), ry = clampi(cry0 + row, 0, mh);
-                    for (int col = 0; col < cw; col++) {
-                        const int sx = clampi(ccx + col, 0, mw), rx = clampi(crx0 + col, 0, mw);
-                        const int64_t si = (int64_t)(b.c_off + sy), ri = (int64_t)(cref + ry);
-                        uv += (uint32_t)abs(rd(b.sp[1], b.sl[1], si * b.ss[1] + sx, 128) - rd(b.rp[1], b.rl[1], ri * b.rs[1] + rx, 128));
-                        uv += (uint32_t)abs(rd(b.sp[2], b.sl[2], si * b.ss[2] + sx, 128) - rd(b.rp[2], b.rl[2], ri * b.rs[2] + rx, 128));
-                    }
-                }
-            }
-            C = sat_mul(uv, 4);
-        }
-        return;
-    }
-    // ---- interior path, luma (block_cost_luma_interior)
+    // interior luma geometry (block_cost_luma_interior)
     const int ipx = b.px - ((b.level > 0 && b.px + b.bw >= b.lw) ? 1 : 0);
     const int ipy = b.py - ((b.level > 0 && b.py + b.bh >= b.lh) ? 1 : 0);
     const int mxp = subpel ? (mvx >> 1) : mvx, myp = subpel ? (mvy >> 1) : mvy;
     const int sub = subpel ? (mvy & 1) * pel + (mvx & 1) : 0;
-    const int rrow = b.y_off + sub * b.lh;
-    const uint8_t* S0 = b.sp[0];
-    const uint8_t* R0 = b.rp[0];
     const int frx = ipx + mxp, fry = ipy + myp;
-    bool done_luma = false;
-    if (ipx >= 0 && ipy >= 0 && ipx + b.bw <= b.lw && ipy + b.bh <= b.lh && frx >= 0 && fry >= 0 &&
-        frx + b.bw <= b.lw && fry + b.bh <= b.lh && (sq || (b.bw == b.bh))) {
-        // interior: satd_interior / sad_u8_simd read exactly these rows
-        done_luma = block_fast_any(b.bw, b.bh, S0 + (int64_t)(b.y_off + ipy) * b.ss[0] + ipx, b.ss[0],
-                                   R0 + (int64_t)(rrow + fry) * b.rs[0] + frx, b.rs[0], sq, L);
-    }
-    if (done_luma) {
-    } else if (sq) {  // satd_luma_clamp (:2353-2436): source outside the level reads 0, reference clamped
-        uint32_t total = 0;
-        const int n = b.bw;
-        for (int ty = 0; ty < n / 4; ty++)
-            for (int tx = 0; tx < n / 4; tx++) {
-                int d[16];
-#pragma unroll
-                for (int rr = 0; rr < 4; rr++) {
-                    const int sy = ipy + ty * 4 + rr;
-                    const int ry = clampi(sy + myp, 0, b.lh - 1);
-#pragma unroll
-                    for (int c = 0; c < 4; c++) {
-                        const int sx = ipx + tx * 4 + c;
-                        const int rx = clampi(sx + mxp, 0, b.lw - 1);
-                        const int a = (sy >= 0 && sy < b.lh && sx >= 0 && sx < b.lw)
-                                          ? rd(S0, b.sl[0], (int64_t)(b.y_off + sy) * b.ss[0] + sx, 0) : 0;
-                        d[rr * 4 + c] = a - rd(R0, b.rl[0], (int64_t)(rrow + ry) * b.rs[0] + rx, 0);
-                    }
-                }
-                total = sat_add(total, had4(d));
-            }
-        L = total >> 1;
-    } else {  // SAD with the scalar loop's row/column breaks
-        uint32_t sad = 0;
-        for (int row = 0; row < b.bh; row++) {
-            const int sy = ipy + row;
-            if (sy < 0 || sy >= b.lh) break;
-            const int ry = clampi(ipy + row + myp, 0, b.lh - 1);
-            for (int col = 0; col < b.bw; col++) {
-                const int sx = ipx + col;
-                if (sx < 0 || sx >= b.lw) break;
-                const int rx = clampi(ipx + col + mxp, 0, b.lw - 1);
-                sad += (uint32_t)abs(rd(S0, b.sl[0], (int64_t)(b.y_off + sy) * b.ss[0] + sx, 0) -
-                                     rd(R0, b.rl[0], (int64_t)(rrow + ry) * b.rs[0] + rx, 0));
-            }
-        }
-        L = sad;
-    }
-    C = 0;
-    if (!chroma) return;
-    // ---- interior path, chroma (chroma_sad_x4; x86-64 semantics)
+    const bool lfast = !edge && pow2_4_32(b.bw) && b.bw == b.bh && ipx >= 0 && ipy >= 0 && ipx + b.bw <= b.lw &&
+                       ipy + b.bh <= b.lh && frx >= 0 && fry >= 0 && frx + b.bw <= b.lw && fry + b.bh <= b.lh;
+    // interior chroma geometry (chroma_sad_x4)
     const int cw = b.bw / 2, ch = b.bh / 2;
-    if (cw < 4 || ch < 4) return;
+    const bool cuse = chroma && cw >= 4 && ch >= 4;
     const int cpx = b.px / 2, cpy = b.py / 2;
     const int mcx = subpel ? (mvx >> 2) : (mvx >> 1), mcy = subpel ? (mvy >> 2) : (mvy >> 1);
-    const int cref = b.c_off + ((b.level == 0 && pel > 1) ? sub * b.clh : 0);
-    if (cw == 4 || cw == 8 || cw == 16) {
-        const int rx0 = cpx + mcx, ry0 = cpy + mcy;
-        if (cpx + cw <= b.clw && cpy + ch <= b.clh && rx0 >= 0 && ry0 >= 0 && rx0 + cw <= b.clw && ry0 + ch <= b.clh) {
-            uint32_t acc = 0;
-            if (cw == ch) {
-                uint32_t vu, vv;
-                const bool fu = block_fast_any(cw, ch, b.sp[1] + (int64_t)(b.c_off + cpy) * b.ss[1] + cpx, b.ss[1],
-                                               b.rp[1] + (int64_t)(cref + ry0) * b.rs[1] + rx0, b.rs[1], b.satd, vu);
-                const bool fv = block_fast_any(cw, ch, b.sp[2] + (int64_t)(b.c_off + cpy) * b.ss[2] + cpx, b.ss[2],
-                                               b.rp[2] + (int64_t)(cref + ry0) * b.rs[2] + rx0, b.rs[2], b.satd, vv);
-                if (fu && fv) {
-                    C = sat_mul(sat_add(vu, vv), 4);
-                    return;
-                }
-            }
-            for (int pl = 1; pl < 3; pl++) {
-                const int64_t sbase = (int64_t)(b.c_off + cpy) * b.ss[pl] + cpx;
-                const int64_t rbase = (int64_t)(cref + ry0) * b.rs[pl] + rx0;
-                uint32_t v = 0;
-                if (b.satd) {  // satd_interior over cw x cw
-                    for (int ty = 0; ty < cw / 4; ty++)
-                        for (int tx = 0; tx < cw / 4; tx++) {
-                            int d[16];
+    const int crx = cpx + mcx, cry = cpy + mcy;
+    const bool cfast = cw == ch && (cw == 4 || cw == 8 || cw == 16) && cpx + cw <= b.clw && cpy + ch <= b.clh &&
+                       crx >= 0 && cry >= 0 && crx + cw <= b.clw && cry + ch <= b.clh;
+    const bool fast = lfast && (!cuse || cfast);
+    uint32_t a0 = 0, a1 = 0, a2 = 0;  // luma, u, v partial sums (fast) or (L, C, 0) from part 0
+    if (fast) {
+        const int rrow = b.y_off + sub * b.lh;
+        const int cref = b.c_off + ((b.level == 0 && pel > 1) ? sub * b.clh : 0);
+        const uint8_t* ls = b.sp[0] + (int64_t)(b.y_off + ipy) * b.ss[0] + ipx;
+        const uint8_t* lr = b.rp[0] + (int64_t)(rrow + fry) * b.rs[0] + frx;
+        const uint8_t* us = b.sp[1] + (int64_t)(b.c_off + cpy) * b.ss[1] + cpx;
+        const uint8_t* ur = b.rp[1] + (int64_t)(cref + cry) * b.rs[1] + crx;
+        const uint8_t* vs = b.sp[2] + (int64_t)(b.c_off + cpy) * b.ss[2] + cpx;
+        const uint8_t* vr = b.rp[2] + (int64_t)(cref + cry) * b.rs[2] + crx;
+        if (sq) {  // luma 4x4 tiles, then u tiles, then v tiles (chroma SATD whenever luma is square SATD)
+            const int lt = b.bw >> 2, ct = cw >> 2;
+            const int nl = lt * lt, nc = cuse ? ct * ct : 0;
+            for (int u = part; u < nl + 2 * nc; u += kGroup) {
+                const int plane = u < nl ? 0 : (u < nl + nc ? 1 : 2);
+                const int t = plane == 0 ? u : (plane == 1 ? u - nl : u - nl - nc);
+                const int per = plane == 0 ? lt : ct;
+                const int ty = t / per, tx = t - ty * per;
+                const uint8_t* sp = plane == 0 ? ls : (plane == 1 ? us : vs);
+                const uint8_t* rp = plane == 0 ? lr : (plane == 1 ? ur : vr);
+                const int ss = plane == 0 ? b.ss[0] : (plane == 1 ? b.ss[1] : b.ss[2]);
+                const int rs = plane == 0 ? b.rs[0] : (plane == 1 ? b.rs[1] : b.rs[2]);
+                uint32_t x[4], y[4];
 #pragma unroll
-                            for (int rr = 0; rr < 4; rr++)
-#pragma unroll
-                                for (int c = 0; c < 4; c++)
-                                    d[rr * 4 + c] = rd(b.sp[pl], b.sl[pl], sbase + (int64_t)(ty * 4 + rr) * b.ss[pl] + tx * 4 + c, 0) -
-                                                    rd(b.rp[pl], b.rl[pl], rbase + (int64_t)(ty * 4 + rr) * b.rs[pl] + tx * 4 + c, 0);
-                            v = sat_add(v, had4(d));
-                        }
-                    v >>= 1;
-                } else {
-                    for (int rr = 0; rr < ch; rr++)
-                        for (int c = 0; c < cw; c++)
-                            v += (uint32_t)abs(rd(b.sp[pl], b.sl[pl], sbase + (int64_t)rr * b.ss[pl] + c, 0) -
-                                               rd(b.rp[pl], b.rl[pl], rbase + (int64_t)rr * b.rs[pl] + c, 0));
+                for (int k = 0; k < 4; k++) {
+                    x[k] = ld32(sp + (int64_t)(ty * 4 + k) * ss + tx * 4);
+                    y[k] = ld32(rp + (int64_t)(ty * 4 + k) * rs + tx * 4);
                 }
-                acc = sat_add(acc, v);
+                const uint32_t h = had4_rows(x, y);
+                if (plane == 0) a0 += h; else if (plane == 1) a1 += h; else a2 += h;
             }
-            C = sat_mul(acc, 4);
-            return;
+        } else if (!b.satd) {  // SAD rows: luma rows, then u rows, then v rows
+            const int nl = b.bh, nc = cuse ? ch : 0;
+            for (int u = part; u < nl + 2 * nc; u += kGroup) {
+                const int plane = u < nl ? 0 : (u < nl + nc ? 1 : 2);
+                const int r = plane == 0 ? u : (plane == 1 ? u - nl : u - nl - nc);
+                const uint8_t* sp = plane == 0 ? ls : (plane == 1 ? us : vs);
+                const uint8_t* rp = plane == 0 ? lr : (plane == 1 ? ur : vr);
+                const int ss = plane == 0 ? b.ss[0] : (plane == 1 ? b.ss[1] : b.ss[2]);
+                const int rs = plane == 0 ? b.rs[0] : (plane == 1 ? b.rs[1] : b.rs[2]);
+                const int words = (plane == 0 ? b.bw : cw) >> 2;
+                uint32_t acc = 0;
+                for (int w = 0; w < words; w++)
+                    acc = __builtin_amdgcn_sad_u8(ld32(sp + (int64_t)r * ss + 4 * w), ld32(rp + (int64_t)r * rs + 4 * w), acc);
+                if (plane == 0) a0 += acc; else if (plane == 1) a1 += acc; else a2 += acc;
+            }
+        } else {  // SATD requested on a square non-power-of-two size: SAD luma, SATD chroma
+            // (not reached: lfast requires a power-of-two square block, so sq holds whenever satd does)
         }
+    } else if (part == 0) {
+        cand_cost_full(b, mvx, mvy, chroma, a0, a1);
     }
-    uint32_t uv = 0;
-    for (int row = 0; row < ch; row++) {
-        const int sy = cpy + row;
-        if (sy >= b.clh) break;
-        const int ry = clampi(cpy + row + mcy, 0, b.clh - 1);
-        for (int col = 0; col < cw; col++) {
-            const int sx = cpx + col;
-            if (sx >= b.clw) break;
-            const int rx = clampi(cpx + col + mcx, 0, b.clw - 1);
-            const int64_t si = (int64_t)(b.c_off + sy), ri = (int64_t)(cref + ry);
-            uv += (uint32_t)abs(rd(b.sp[1], b.sl[1], si * b.ss[1] + sx, 128) - rd(b.rp[1], b.rl[1], ri * b.rs[1] + rx, 128));
-            uv += (uint32_t)abs(rd(b.sp[2], b.sl[2], si * b.ss[2] + sx, 128) - rd(b.rp[2], b.rl[2], ri * b.rs[2] + rx, 128));
-        }
+    a0 = xor_sum8(a0);
+    a1 = xor_sum8(a1);
+    a2 = xor_sum8(a2);
+    if (fast) {
+        Lout = sq ? (a0 >> 1) : a0;
+        const uint32_t U = b.satd ? (a1 >> 1) : a1, V = b.satd ? (a2 >> 1) : a2;
+        Cout = cuse ? sat_mul(sat_add(U, V), 4) : 0;
+    } else {
+        Lout = a0;
+        Cout = a1;
     }
-    C = sat_mul(uv, 4);
 }
 
 struct Best {
@@ -352,99 +191,21 @@ struct Search {
     int bnd[4];
     int lambda, pnew;
     bool chroma;
-    bool serial;  // pnew < -256: replay the reference's skip rules lane by lane
+    bool serial;  // pnew < -256: replay the reference's skip rules candidate by candidate
 };
 
 __device__ __forceinline__ bool in_bounds(const Search& s, int x, int y) {
     return x >= s.bnd[0] && y >= s.bnd[1] && x < s.bnd[2] && y < s.bnd[3];
 }
 
-__device__ __forceinline__ void wave_argmin(int64_t& c, int& k) {
+// Lowest (cost, index) across the groups of the wave (values are uniform per group).
+__device__ __forceinline__ void group_argmin(int64_t& c, int& k) {
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
+    for (int off = kGroup; off < kWave; off <<= 1) {
         const int64_t oc = __shfl_xor(c, off);
         const int ok = __shfl_xor(k, off);
         if (oc < c || (oc == c && ok < k)) { c = oc; k = ok; }
     }
-}
-
-// exact_check over candidates 0..n-1 in list order (gen(k, &x, &y) yields candidate k).
-// Updates `best` as the reference's sequence of exact_check calls would and returns
-// the index of the candidate that last improved it, or -1.
-template <class Gen>
-__device__ __attribute__((always_inline)) int check_list(const Blk& b, const Search& s, int n, Gen gen, Best& best) {
-    const int lane = threadIdx.x;
-    int win = -1;
-    for (int base = 0; base < n; base += kWave) {
-        const int k = base + lane;
-        int64_t cost = INT64_MAX, motion = INT64_MAX;
-        uint32_t sad = 0;
-        int mx = 0, my = 0;
-        bool valid = false;
-        if (k < n) {
-            gen(k, mx, my);
-            if (in_bounds(s, mx, my)) {
-                valid = true;
-                const int64_t dx = s.pmx - mx, dy = s.pmy - my;
-                motion = ((int64_t)s.lambda * (dx * dx + dy * dy)) >> 8;
-                uint32_t l, c;
-                cand_cost(b, mx, my, s.chroma, l, c);
-                sad = sat_add(l, c);
-                cost = motion + (int64_t)l + (((int64_t)s.pnew * (int64_t)l) >> 8) + (int64_t)c +
-                       (((int64_t)s.pnew * (int64_t)c) >> 8);
-            }
-        }
-        if (!s.serial) {
-            int64_t mc = valid ? cost : INT64_MAX;
-            int mk = valid ? k : 0x7FFFFFFF;
-            wave_argmin(mc, mk);
-            if (mk != 0x7FFFFFFF && mc < best.cost) {
-                const int src = mk - base;
-                best.mvx = __shfl(mx, src);
-                best.mvy = __shfl(my, src);
-                best.sad = __shfl(sad, src);
-                best.cost = mc;
-                win = mk;
-            }
-        } else {
-            const int cnt = min(kWave, n - base);
-            for (int j = 0; j < cnt; j++) {
-                if (!__shfl((int)valid, j)) continue;
-                if (__shfl(motion, j) >= best.cost) continue;
-                const int64_t cj = __shfl(cost, j);
-                if (cj < best.cost) {
-                    best.mvx = __shfl(mx, j);
-                    best.mvy = __shfl(my, j);
-                    best.sad = __shfl(sad, j);
-                    best.cost = cj;
-                    win = base + j;
-                }
-            }
-        }
-    }
-    return win;
-}
-
-// exact_expanding (:829-886): square ring of `radius` around (cx, cy) with `step`.
-__device__ __attribute__((always_inline)) void expanding(const Blk& b, const Search& s, int cx, int cy, int r, int step, Best& best) {
-    const int m = (2 * r - 1) / step;  // values of i in -r+step, -r+2step, ... < r
-    const int n = 4 * m + 4;
-    check_list(b, s, n,
-               [&](int k, int& x, int& y) {
-                   if (k < 2 * m) {
-                       x = cx - r + step * (1 + k / 2);
-                       y = (k & 1) ? cy + r : cy - r;
-                   } else if (k < 4 * m) {
-                       const int j = k - 2 * m;
-                       y = cy - r + step * (1 + j / 2);
-                       x = (j & 1) ? cx + r : cx - r;
-                   } else {
-                       const int c = k - 4 * m;  // (-r,-r), (-r,+r), (+r,-r), (+r,+r)
-                       x = (c < 2) ? cx - r : cx + r;
-                       y = (c & 1) ? cy + r : cy - r;
-                   }
-               },
-               best);
 }
 
 __constant__ int kHex2[8][2] = {{-1, -2}, {-2, 0}, {-1, 2}, {1, 2}, {2, 0}, {1, -2}, {-1, -2}, {-2, 0}};
@@ -452,68 +213,69 @@ __constant__ int kMod6m1[8] = {5, 0, 1, 2, 3, 4, 5, 0};
 __constant__ int kHex4[16][2] = {{-4, 2}, {-4, 1}, {-4, 0}, {-4, -1}, {-4, -2}, {4, -2}, {4, -1}, {4, 0},
                                  {4, 1},  {4, 2},  {2, 3},  {0, 4},   {-2, 3},  {-2, -3}, {0, -4}, {2, -3}};
 
-// exact_hex2 (:941-1033).
-__device__ __attribute__((always_inline)) void hex2(const Blk& b, const Search& s, int radius, Best& best) {
-    int bmx = best.mvx, bmy = best.mvy;
-    if (radius > 1) {
-        int dir = -2;
-        int w = check_list(b, s, 6, [&](int k, int& x, int& y) { x = bmx + kHex2[k + 1][0]; y = bmy + kHex2[k + 1][1]; }, best);
-        if (w >= 0) dir = w;
-        if (dir != -2) {
-            bmx += kHex2[dir + 1][0];
-            bmy += kHex2[dir + 1][1];
-            int i = 1;
-            while (i < radius / 2 && in_bounds(s, bmx, bmy)) {
-                const int odir = kMod6m1[dir + 1];
-                dir = -2;
-                w = check_list(b, s, 3, [&](int k, int& x, int& y) { x = bmx + kHex2[odir + k][0]; y = bmy + kHex2[odir + k][1]; }, best);
-                if (w >= 0) dir = odir + w - 1;
-                if (dir == -2) break;
-                bmx += kHex2[dir + 1][0];
-                bmy += kHex2[dir + 1][1];
-                i++;
-            }
+// Candidate lists of the reference's search steps, in its evaluation order.
+enum PatKind : int {
+    kPatPred = 0,   // predictor list (lanes hold the candidates)
+    kPatExpand,     // exact_expanding (:829-886): square ring of radius r, step
+    kPatHexFirst,   // exact_hex2 first step (:941-1033): 6 points
+    kPatHexStep,    // exact_hex2 iteration: 3 points from odir
+    kPatUmh,        // exact_umh (:1036-1119): cross, then 16-point hexagons
+    kPatExh,        // exhaustive (2r+1)^2 window, raster order (:771-826)
+};
+
+struct Pattern {
+    int kind, cx, cy, r, step, n;
+};
+
+__device__ __forceinline__ int pat_count(int kind, int r, int step) {
+    if (kind == kPatExpand) return 4 * ((2 * r - 1) / step) + 4;
+    if (kind == kPatHexFirst) return 6;
+    if (kind == kPatHexStep) return 3;
+    if (kind == kPatUmh) return 4 * (r / 2) + 16 * max(r / 4, 1);
+    return (2 * r + 1) * (2 * r + 1);  // kPatExh
+}
+
+__device__ __forceinline__ void pat_gen(const Pattern& p, int k, int& x, int& y) {
+    const int cx = p.cx, cy = p.cy, r = p.r;
+    if (p.kind == kPatExpand) {
+        const int m = (2 * r - 1) / p.step;  // values of i in -r+step, -r+2step, ... < r
+        if (k < 2 * m) {
+            x = cx - r + p.step * (1 + k / 2);
+            y = (k & 1) ? cy + r : cy - r;
+        } else if (k < 4 * m) {
+            const int j = k - 2 * m;
+            y = cy - r + p.step * (1 + j / 2);
+            x = (j & 1) ? cx + r : cx - r;
+        } else {
+            const int c = k - 4 * m;  // (-r,-r), (-r,+r), (+r,-r), (+r,+r)
+            x = (c < 2) ? cx - r : cx + r;
+            y = (c & 1) ? cy + r : cy - r;
         }
-        best.mvx = bmx;
-        best.mvy = bmy;
-    }
-    expanding(b, s, bmx, bmy, 1, 1, best);
-}
-
-// exact_umh (:1036-1119): cross, multi-scale 16-point hexagon, then hex2.
-__device__ __attribute__((always_inline)) void umh(const Blk& b, const Search& s, int cx, int cy, int radius, Best& best) {
-    const int nd = radius / 2;                   // odd d in [1, radius)
-    const int ns = max(radius / 4, 1);
-    const int n = 4 * nd + 16 * ns;
-    check_list(b, s, n,
-               [&](int k, int& x, int& y) {
-                   if (k < 2 * nd) {
-                       const int d = 1 + 2 * (k / 2);
-                       x = (k & 1) ? cx + d : cx - d;
-                       y = cy;
-                   } else if (k < 4 * nd) {
-                       const int j = k - 2 * nd, d = 1 + 2 * (j / 2);
-                       x = cx;
-                       y = (j & 1) ? cy + d : cy - d;
-                   } else {
-                       const int j = k - 4 * nd, sc = 1 + j / 16, h = j % 16;
-                       x = cx + kHex4[h][0] * sc;
-                       y = cy + kHex4[h][1] * sc;
-                   }
-               },
-               best);
-    hex2(b, s, radius, best);
-}
-
-// exact_refine_search (:771-826).
-__device__ __attribute__((always_inline)) void refine(const Blk& b, const Search& s, int type, int radius, Best& best) {
-    if (type == 2) {
-        hex2(b, s, radius, best);
-    } else if (type == 3) {
-        umh(b, s, best.mvx, best.mvy, radius, best);
-    } else if (type == 4) {
-        const int cx = best.mvx, cy = best.mvy, w = 2 * radius + 1;
-        check_list(b, s, w * w, [&](int k, int& x, int& y) { x = cx - radius + k % w; y = cy - radius + k / w; }, best);
+    } else if (p.kind == kPatHexFirst) {
+        x = cx + kHex2[k + 1][0];
+        y = cy + kHex2[k + 1][1];
+    } else if (p.kind == kPatHexStep) {  // p.step holds odir
+        x = cx + kHex2[p.step + k][0];
+        y = cy + kHex2[p.step + k][1];
+    } else if (p.kind == kPatUmh) {
+        const int nd = r / 2;
+        if (k < 2 * nd) {
+            const int d = 1 + 2 * (k / 2);
+            x = (k & 1) ? cx + d : cx - d;
+            y = cy;
+        } else if (k < 4 * nd) {
+            const int j = k - 2 * nd, d = 1 + 2 * (j / 2);
+            x = cx;
+            y = (j & 1) ? cy + d : cy - d;
+        } else {
+            const int j = k - 4 * nd, sc = 1 + j / 16, h = j % 16;
+            x = cx + kHex4[h][0] * sc;
+            y = cy + kHex4[h][1] * sc;
+        }
+    } else {  // kPatExh
+        const int w = 2 * r + 1;
+        x = cx - r + k % w;
+        y = cy - r + k / w;
     }
 }
 
@@ -582,188 +344,585 @@ __device__ __attribute__((always_inline)) uint32_t plane_activity(const uint8_t*
     return (acc >> 1) / area;
 }
 
+// A wave-uniform struct read into scalar registers: every lane loads the same words
+// and readfirstlane marks them uniform (the kernels also store to global memory, so
+// the compiler cannot prove the descriptors unclobbered and would keep them in VGPRs).
+template <class T>
+__device__ __forceinline__ T uniform_load(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "word-sized");
+    T v;
+    const int* src = reinterpret_cast<const int*>(p);
+    int* dst = reinterpret_cast<int*>(&v);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); i++) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    return v;
+}
+__device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// A result written earlier by this wave (per lane; the lanes may read different blocks).
 __device__ __forceinline__ DVec load_vec(const DVec* p) {
-    // Field entries written earlier by this wave: make them visible before reading.
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     return *p;
 }
 
-// exact_search_level (:480-681) for one chain: one workgroup = one wavefront.
-__global__ void __launch_bounds__(kWave) search_level_kernel(const Chain* __restrict__ chains) {
-    const Chain ch = chains[blockIdx.x];
-    const LevelGeom g = *ch.geom;
-    const int lane = threadIdx.x;
-    const int gx = ch.global ? ch.global[0] : 0, gy = ch.global ? ch.global[1] : 0;
-    Blk b;
+__device__ __forceinline__ void init_blk(const Chain& ch, const LevelGeom& g, Blk& b) {
     for (int i = 0; i < 3; i++) {
         b.sp[i] = ch.src.p[i]; b.rp[i] = ch.ref.p[i];
         b.ss[i] = ch.src.stride[i]; b.rs[i] = ch.ref.stride[i];
-        b.sl[i] = ch.src.len[i]; b.rl[i] = ch.ref.len[i];
+        b.sl[i] = (int)ch.src.len[i]; b.rl[i] = (int)ch.ref.len[i];
     }
     b.level = g.level; b.lw = g.lw; b.lh = g.lh; b.y_off = g.y_off; b.c_off = g.c_off;
     b.clw = g.clw; b.clh = g.clh; b.pel = g.pel; b.satd = g.satd != 0;
+    b.px = 0; b.py = 0; b.bw = g.bw; b.bh = g.bh;
+}
+
+// Steps of one block's search (exact_search_block :684-768, the refine searches
+// :771-1119, the coarse bad-block pass :607-672).  A step names what to do with the
+// result of the candidate list just evaluated; every list goes through the single
+// evaluation site of search_block.
+enum Step : int {
+    kStEnd = 0,
+    kStEdgeDone,      // edge row/column: the predictor's score measured at its source block
+    kStPredDone,      // predictor candidates evaluated
+    kStUmhDone,       // exact_umh's cross + hexagons evaluated: continue with exact_hex2
+    kStHexBegin,      // exact_hex2 entry (no list)
+    kStHexFirstDone,
+    kStHexLoop,       // (no list)
+    kStHexStepDone,
+    kStExp1Done,      // exact_hex2's closing expanding(1, 1) evaluated
+    kStExhDone,
+    kStRefineDone,    // (no list) bad-block decision
+    kStBadRing,       // (no list)
+    kStBadRingDone,
+    kStBadFinal,      // (no list)
+    kStBadFinalDone,
+};
+
+// One block of exact_search_level (:512-680).  `in` holds the level's interpolated
+// predictors and `res` the results; the 3x3 neighbours visited before this block
+// (opos below its own) seed the search and, on the edge row/column, give the
+// predictor -- exactly what the reference's `processed` grid selects in its visit
+// order.  Returns the block's vector (uniform across the wave), *bad = whether the
+// coarse-level bad-block expansion ran and *found_out = the cost before it.
+__device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g, Blk& b, int idx, int bad_count, bool* bad,
+                             uint32_t* found_out) {
+    const int lane = threadIdx.x;
+    const int part = lane & (kGroup - 1);
+    const int gx = ch.global ? uniform_int(ch.global[0]) : 0, gy = ch.global ? uniform_int(ch.global[1]) : 0;
     const int gw = g.gw, gh = g.gh, level = g.level;
     const bool smallest = g.smallest != 0;
-    int bad_count = 0;
-    const int nblocks = gw * gh;
-    for (int it = 0; it < nblocks; it++) {
-        const int packed = ch.order[it];
-        const int bx = packed & 0xFFFF, by = packed >> 16;
-        if (bx >= gw || by >= gh) continue;  // defensive: orders always hold in-grid positions
-        const int idx = by * gw + bx;
-        int px = (level > 0 && bx == gw - 1) ? g.lw - g.bw : bx * g.step_x;
-        int py = (level > 0 && by == gh - 1) ? g.lh - g.bh : by * g.step_y;
-        int bw = g.bw, bh = g.bh;
-        if (smallest) { px += g.bw >> 2; py += g.bh >> 2; bw = g.bw >> 1; bh = g.bh >> 1; }
-        b.px = px; b.py = py; b.bw = bw; b.bh = bh;
-        const bool edge = level > 0 && (bx == gw - 1 || by == gh - 1);
-        const int sx = min(bx, gw - 2), sy = min(by, gh - 2);
-        DVec pred = load_vec(&ch.field[sy * gw + sx]);
-        if (edge && !smallest) {
-            // predictor score re-measured at the source block's own position
-            Blk e = b;
-            e.px = sx * g.step_x;
-            e.py = sy * g.step_y;
-            uint32_t l, c;
-            cand_cost(e, pred.dx, pred.dy, true, l, c);
-            pred.score = sat_add(l, c);
-        } else if (!edge) {
-            pred = load_vec(&ch.field[idx]);
-        }
-        const int lambda = adapt_lambda(smallest ? 0 : g.lambda_base, g.lsad, pred.score);
-        int radius = g.radius;
-        if (level > 0 && g.adaptive) {
-            const int ax = px - ((px + bw >= g.lw) ? 1 : 0), ay = py - ((py + bh >= g.lh) ? 1 : 0);
-            uint32_t act = plane_activity(b.sp[0], b.ss[0], g.y_off + ay, ax, bw, bh, b.satd) +
-                           plane_activity(b.sp[1], b.ss[1], g.c_off + py / 2, px / 2, bw / 2, bh / 2, b.satd) +
-                           plane_activity(b.sp[2], b.ss[2], g.c_off + py / 2, px / 2, bw / 2, bh / 2, b.satd);
-            radius = act < 2 ? 0 : ((radius * (int)min(255u * act / 150u, 255u)) >> 8) + 1;
-        }
-        // seeds: vectors of already-searched 3x3 neighbours, row-major (:574-584)
-        int nseed_lane = 0;
-        int sdx = 0, sdy = 0;
-        if (lane < 9) {
-            const int nx = bx - 1 + lane % 3, ny = by - 1 + lane / 3;
-            if (nx >= 0 && ny >= 0 && nx < gw && ny < gh) {
-                const DVec nb = load_vec(&ch.field[ny * gw + nx]);
-                if (nb.done) { nseed_lane = 1; sdx = nb.dx; sdy = nb.dy; }
-            }
-        }
-        const uint64_t smask = __ballot(nseed_lane);
-        const int nseeds = __popcll(smask);
-
-        Search s;
-        s.pmx = pred.dx;
-        s.pmy = pred.dy;
-        s.pnew = g.pnew;
-        s.chroma = !smallest;
-        s.serial = g.pnew < -256;
-        s.bnd[0] = -g.pel * px; s.bnd[1] = -g.pel * py;
-        s.bnd[2] = g.pel * (g.lw - px - bw) + 1; s.bnd[3] = g.pel * (g.lh - py - bh) + 1;
-        s.lambda = lambda;
-
-        // ---- exact_search_block (:684-768): predictor candidates across lanes
-        const bool field_valid = pred.score != 0xFFFFFFFFu;
-        bool has_prev = false;
-        int prx = 0, pry = 0;
-        if (ch.prev) {
-            const DVec pv = ch.prev[idx];
-            has_prev = pv.score != 0xFFFFFFFFu && pv.score > 0;
-            prx = pv.dx; pry = pv.dy;
-        }
-        // lane 0 field, 1 prev, 2 zero, 3 global, 4.. seeds
-        int cmx = 0, cmy = 0, pen = 0;
-        bool active = false;
-        if (lane == 0) { active = field_valid; cmx = pred.dx; cmy = pred.dy; pen = 0; }
-        else if (lane == 1) { active = has_prev; cmx = prx; cmy = pry; pen = g.pprev; }
-        else if (lane == 2) { active = true; pen = g.pzero; }
-        else if (lane == 3) { active = true; cmx = gx; cmy = gy; pen = g.pglobal; }
-        else if (lane < 4 + nseeds) { active = true; pen = g.pnbour; }
-        {  // seed j lives in the lane of the j-th set bit of the ballot
-            const int j = lane - 4;
-            int src = 0, cnt = 0;
-            for (int q = 0; q < 9; q++)
-                if ((smask >> q) & 1) { if (cnt == j) src = q; cnt++; }
-            const int tdx = __shfl(sdx, src), tdy = __shfl(sdy, src);
-            if (lane >= 4 && lane < 4 + nseeds) { cmx = tdx; cmy = tdy; }
-        }
-        uint32_t csad = 0;
-        int64_t ccost = 0;
-        if (active) {
-            uint32_t l, c;
-            cand_cost(b, cmx, cmy, !smallest, l, c);
-            csad = sat_add(l, c);
-            ccost = (int64_t)csad + (((int64_t)pen * (int64_t)csad) >> 8);
-        }
-        Best best;
-        Search r = s;
-        r.pmx = pred.dx;
-        r.pmy = pred.dy;
-        if (g.early) {
-            // level 0 with distance 0: the predictor (or zero) is the answer
-            const int src = field_valid ? 0 : 2;
-            best.mvx = __shfl(cmx, src);
-            best.mvy = __shfl(cmy, src);
-            best.sad = __shfl(csad, src);
-            best.cost = (int64_t)best.sad;
+    const int bx = idx % gw, by = idx / gw;
+    const int me = uniform_int(ch.opos[idx]);
+    int px = (level > 0 && bx == gw - 1) ? g.lw - g.bw : bx * g.step_x;
+    int py = (level > 0 && by == gh - 1) ? g.lh - g.bh : by * g.step_y;
+    int bw = g.bw, bh = g.bh;
+    if (smallest) { px += g.bw >> 2; py += g.bh >> 2; bw = g.bw >> 1; bh = g.bh >> 1; }
+    b.px = px; b.py = py; b.bw = bw; b.bh = bh;
+    const bool edge = level > 0 && (bx == gw - 1 || by == gh - 1);
+    DVec pred;
+    int epx = 0, epy = 0;
+    if (edge) {
+        const int si = min(by, gh - 2) * gw + min(bx, gw - 2);
+        if (uniform_int(ch.opos[si]) < me) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            pred = uniform_load(&ch.res[si]);
         } else {
-            if (field_valid) {
-                best = Best{pred.dx, pred.dy, __shfl(csad, 0), __shfl(ccost, 0)};
-            } else {
-                best = Best{0, 0, 0xFFFFFFFFu, INT64_MAX};
-            }
-            if (has_prev) {
-                const int64_t pc = __shfl(ccost, 1);
-                if (pc < best.cost) {
-                    best = Best{prx, pry, __shfl(csad, 1), pc};
-                    r.pmx = prx;
-                    r.pmy = pry;
+            pred = uniform_load(&ch.in[si]);
+        }
+        epx = (si % gw) * g.step_x;
+        epy = (si / gw) * g.step_y;
+    } else {
+        pred = uniform_load(&ch.in[idx]);
+    }
+    int radius = g.radius;
+    if (level > 0 && g.adaptive) {
+        const int ax = px - ((px + bw >= g.lw) ? 1 : 0), ay = py - ((py + bh >= g.lh) ? 1 : 0);
+        uint32_t act = plane_activity(b.sp[0], b.ss[0], g.y_off + ay, ax, bw, bh, b.satd) +
+                       plane_activity(b.sp[1], b.ss[1], g.c_off + py / 2, px / 2, bw / 2, bh / 2, b.satd) +
+                       plane_activity(b.sp[2], b.ss[2], g.c_off + py / 2, px / 2, bw / 2, bh / 2, b.satd);
+        radius = act < 2 ? 0 : ((radius * (int)min(255u * act / 150u, 255u)) >> 8) + 1;
+    }
+    // seeds: vectors of the 3x3 neighbours visited earlier, row-major (:574-584)
+    int nseed_lane = 0;
+    int sdx = 0, sdy = 0;
+    if (lane < 9) {
+        const int nx = bx - 1 + lane % 3, ny = by - 1 + lane / 3;
+        if (nx >= 0 && ny >= 0 && nx < gw && ny < gh && ch.opos[ny * gw + nx] < me) {
+            const DVec nb = load_vec(&ch.res[ny * gw + nx]);
+            nseed_lane = 1; sdx = nb.dx; sdy = nb.dy;
+        }
+    }
+    const uint64_t smask = __ballot(nseed_lane);
+    const int nseeds = __popcll(smask);
+    int seed_x, seed_y;  // seed j in lane 4 + j
+    {
+        const int j = lane - 4;
+        int src = 0, cnt = 0;
+        for (int q = 0; q < 9; q++)
+            if ((smask >> q) & 1) { if (cnt == j) src = q; cnt++; }
+        seed_x = __shfl(sdx, src);
+        seed_y = __shfl(sdy, src);
+    }
+    bool has_prev = false;
+    int prx = 0, pry = 0;
+    if (ch.prev) {
+        const DVec pv = uniform_load(&ch.prev[idx]);
+        has_prev = pv.score != 0xFFFFFFFFu && pv.score > 0;
+        prx = pv.dx; pry = pv.dy;
+    }
+
+    Search s;
+    s.pnew = g.pnew;
+    s.chroma = !smallest;
+    s.serial = g.pnew < -256;
+    s.bnd[0] = -g.pel * px; s.bnd[1] = -g.pel * py;
+    s.bnd[2] = g.pel * (g.lw - px - bw) + 1; s.bnd[3] = g.pel * (g.lh - py - bh) + 1;
+    s.pmx = pred.dx; s.pmy = pred.dy;
+    s.lambda = 0;
+
+    // candidate list held in lanes (lane k = candidate k) for the list steps
+    int lx = 0, ly = 0, lpen = 0, lact = 0;
+    uint32_t lsad = 0;  // list candidate `lane`'s sad after evaluation
+    int64_t lcost = 0;
+    bool field_valid = false;
+    auto set_pred_list = [&]() {
+        // the reference's comparison order (:700-740): field, prev, global, zero, seeds
+        field_valid = pred.score != 0xFFFFFFFFu;
+        lact = 0; lx = 0; ly = 0; lpen = 0;
+        if (lane == 0) { lact = field_valid; lx = pred.dx; ly = pred.dy; }
+        else if (lane == 1) { lact = has_prev; lx = prx; ly = pry; lpen = g.pprev; }
+        else if (lane == 2) { lact = 1; lx = gx; ly = gy; lpen = g.pglobal; }
+        else if (lane == 3) { lact = 1; lpen = g.pzero; }
+        else if (lane < 4 + nseeds) { lact = 1; lx = seed_x; ly = seed_y; lpen = g.pnbour; }
+    };
+
+    Best best{0, 0, 0xFFFFFFFFu, INT64_MAX};
+    Search cur = s;
+    Pattern P;
+    bool pchroma;
+    int step;
+    if (edge && !smallest) {
+        b.px = epx; b.py = epy;
+        lact = lane == 0; lx = pred.dx; ly = pred.dy; lpen = 0;
+        P = Pattern{kPatPred, 0, 0, 0, 0, 1};
+        pchroma = true;
+        step = kStEdgeDone;
+    } else {
+        set_pred_list();
+        P = Pattern{kPatPred, 0, 0, 0, 0, 4 + nseeds};
+        pchroma = !smallest;
+        step = kStPredDone;
+    }
+    int bmx = 0, bmy = 0, dir = -2, hi = 0, odir = 0, hex_r = 0, exp_next = kStRefineDone, ring = 0;
+    uint32_t found = 0;
+    bool is_bad = false;
+    for (;;) {
+        // ---------------- the evaluation site: kPerPass candidates per pass, one group each
+        int win = -1;
+        {
+            const bool list = P.kind == kPatPred;
+            for (int base = 0; base < P.n; base += kPerPass) {
+                const int k = base + (lane >> 3);
+                const int kl = min(k, kWave - 1);
+                const int tx = __shfl(lx, kl), ty = __shfl(ly, kl), tact = __shfl(lact, kl), tpen = __shfl(lpen, kl);
+                int mx = 0, my = 0;
+                bool valid = k < P.n;
+                if (list) {
+                    mx = tx; my = ty;
+                    valid = valid && tact != 0;
+                } else if (valid) {
+                    pat_gen(P, k, mx, my);
+                    valid = in_bounds(cur, mx, my);
+                }
+                uint32_t L = 0, C = 0;
+                if (valid) group_cost(b, mx, my, pchroma, part, L, C);
+                const uint32_t sad = sat_add(L, C);
+                int64_t cost = INT64_MAX, motion = INT64_MAX;
+                if (valid) {
+                    if (list) {
+                        cost = (int64_t)sad + (((int64_t)tpen * (int64_t)sad) >> 8);
+                    } else {
+                        const int64_t dx = cur.pmx - mx, dy = cur.pmy - my;
+                        motion = ((int64_t)cur.lambda * (dx * dx + dy * dy)) >> 8;
+                        cost = motion + (int64_t)L + (((int64_t)cur.pnew * (int64_t)L) >> 8) + (int64_t)C +
+                               (((int64_t)cur.pnew * (int64_t)C) >> 8);
+                    }
+                }
+                if (list) {  // each list candidate's sad and cost back into its own lane
+                    const int from = ((lane - base) & (kPerPass - 1)) * kGroup;
+                    const uint32_t t_sad = __shfl(sad, from);
+                    const int64_t t_cost = __shfl(cost, from);
+                    if (lane >= base && lane < base + kPerPass) { lsad = t_sad; lcost = t_cost; }
+                }
+                if (!cur.serial || list) {
+                    // lowest (cost, index) == the reference's running strict-`<` minimum
+                    int64_t mc = valid ? cost : INT64_MAX;
+                    int mk = valid ? k : 0x7FFFFFFF;
+                    group_argmin(mc, mk);
+                    mc = rl64(mc, 0);
+                    mk = rl(mk, 0);
+                    if (mk != 0x7FFFFFFF && mc < best.cost) {
+                        const int src = (mk - base) * kGroup;
+                        best.mvx = rl(mx, src);
+                        best.mvy = rl(my, src);
+                        best.sad = rlu(sad, src);
+                        best.cost = mc;
+                        win = mk;
+                    }
+                } else {  // pnew < -256: exact_check's skip rules, candidate by candidate
+                    const int cnt = min(kPerPass, P.n - base);
+                    for (int j = 0; j < cnt; j++) {
+                        const int src = j * kGroup;
+                        if (!rl((int)valid, src)) continue;
+                        if (rl64(motion, src) >= best.cost) continue;
+                        const int64_t cj = rl64(cost, src);
+                        if (cj < best.cost) {
+                            best.mvx = rl(mx, src);
+                            best.mvy = rl(my, src);
+                            best.sad = rlu(sad, src);
+                            best.cost = cj;
+                            win = base + j;
+                        }
+                    }
                 }
             }
-            {
-                const int64_t c3 = __shfl(ccost, 3);
-                if (c3 < best.cost) best = Best{gx, gy, __shfl(csad, 3), c3};
-                const int64_t c2 = __shfl(ccost, 2);
-                if (c2 < best.cost) best = Best{0, 0, __shfl(csad, 2), c2};
-            }
-            for (int j = 0; j < nseeds; j++) {
-                const int64_t cj = __shfl(ccost, 4 + j);
-                if (cj < best.cost) best = Best{__shfl(cmx, 4 + j), __shfl(cmy, 4 + j), __shfl(csad, 4 + j), cj};
-            }
-            best.cost = (int64_t)best.sad;
-            const int stype = (edge && level == 1) ? 0 : g.search_type;
-            refine(b, r, stype, radius, best);
         }
-        // ---- bad-block expansion at coarse levels (:607-672), around the zero vector
-        const uint32_t found = best.sad;
-        const int bad_limit = sat_add_i32(g.bad_sad, sat_mul_i32(g.bad_sad, bad_count) / 16);
-        if (level > 0 && idx > 1 && g.bad_sad > 0 && (int)best.sad > bad_limit) {
-            bad_count++;
-            Search e = s;
-            e.pmx = pred.dx;
-            e.pmy = pred.dy;
-            if (g.bad_range < 0) {
-                const int max_r = -g.bad_range * g.pel;
-                for (int rr = g.pel; rr < max_r; rr += g.pel) {
-                    expanding(b, e, 0, 0, rr, g.pel, best);
-                    if (best.sad < found / 4) break;
+        // ---------------- act on the result; pick the next list (or finish)
+        bool next = false;
+        while (!next && step != kStEnd) {
+            switch (step) {
+                case kStEdgeDone:
+                    pred.score = rlu(lsad, 0);
+                    b.px = px; b.py = py;
+                    set_pred_list();
+                    best = Best{0, 0, 0xFFFFFFFFu, INT64_MAX};
+                    P = Pattern{kPatPred, 0, 0, 0, 0, 4 + nseeds};
+                    pchroma = !smallest;
+                    step = kStPredDone;
+                    next = true;
+                    break;
+                case kStPredDone: {
+                    s.lambda = adapt_lambda(smallest ? 0 : g.lambda_base, g.lsad, pred.score);
+                    if (g.early) {
+                        // level 0 with distance 0: the predictor (or zero) is the answer
+                        const int src = field_valid ? 0 : 3;
+                        best.mvx = rl(lx, src);
+                        best.mvy = rl(ly, src);
+                        best.sad = rlu(lsad, src);
+                        best.cost = (int64_t)best.sad;
+                        found = best.sad;
+                        step = kStEnd;
+                        break;
+                    }
+                    cur = s;
+                    if (has_prev) {
+                        const int64_t fc = rl64(lcost, 0), pc = rl64(lcost, 1);
+                        if (pc < (field_valid ? fc : INT64_MAX)) { cur.pmx = prx; cur.pmy = pry; }
+                    }
+                    best.cost = (int64_t)best.sad;
+                    const int stype = (edge && level == 1) ? 0 : g.search_type;
+                    exp_next = kStRefineDone;
+                    if (stype == 2) {
+                        hex_r = radius;
+                        step = kStHexBegin;
+                    } else if (stype == 3) {
+                        hex_r = radius;
+                        P = Pattern{kPatUmh, best.mvx, best.mvy, radius, 0, pat_count(kPatUmh, radius, 0)};
+                        step = kStUmhDone;
+                        next = true;
+                    } else if (stype == 4) {
+                        P = Pattern{kPatExh, best.mvx, best.mvy, radius, 0, pat_count(kPatExh, radius, 0)};
+                        step = kStExhDone;
+                        next = true;
+                    } else {
+                        step = kStRefineDone;
+                    }
+                    break;
                 }
-            } else if (g.bad_range > 0) {
-                umh(b, e, 0, 0, g.bad_range * g.pel, best);
+                case kStUmhDone:
+                    step = kStHexBegin;
+                    break;
+                case kStHexBegin:
+                    bmx = best.mvx;
+                    bmy = best.mvy;
+                    if (hex_r > 1) {
+                        dir = -2;
+                        P = Pattern{kPatHexFirst, bmx, bmy, 0, 0, 6};
+                        step = kStHexFirstDone;
+                    } else {
+                        P = Pattern{kPatExpand, bmx, bmy, 1, 1, pat_count(kPatExpand, 1, 1)};
+                        step = kStExp1Done;
+                    }
+                    next = true;
+                    break;
+                case kStHexFirstDone:
+                    if (win >= 0) {
+                        dir = win;
+                        bmx += kHex2[dir + 1][0];
+                        bmy += kHex2[dir + 1][1];
+                        hi = 1;
+                        step = kStHexLoop;
+                    } else {
+                        best.mvx = bmx;
+                        best.mvy = bmy;
+                        P = Pattern{kPatExpand, bmx, bmy, 1, 1, pat_count(kPatExpand, 1, 1)};
+                        step = kStExp1Done;
+                        next = true;
+                    }
+                    break;
+                case kStHexLoop:
+                    if (hi < hex_r / 2 && in_bounds(cur, bmx, bmy)) {
+                        odir = kMod6m1[dir + 1];
+                        dir = -2;
+                        P = Pattern{kPatHexStep, bmx, bmy, 0, odir, 3};
+                        step = kStHexStepDone;
+                    } else {
+                        best.mvx = bmx;
+                        best.mvy = bmy;
+                        P = Pattern{kPatExpand, bmx, bmy, 1, 1, pat_count(kPatExpand, 1, 1)};
+                        step = kStExp1Done;
+                    }
+                    next = true;
+                    break;
+                case kStHexStepDone:
+                    if (win >= 0) {
+                        dir = odir + win - 1;
+                        bmx += kHex2[dir + 1][0];
+                        bmy += kHex2[dir + 1][1];
+                        hi++;
+                        step = kStHexLoop;
+                    } else {
+                        best.mvx = bmx;
+                        best.mvy = bmy;
+                        P = Pattern{kPatExpand, bmx, bmy, 1, 1, pat_count(kPatExpand, 1, 1)};
+                        step = kStExp1Done;
+                        next = true;
+                    }
+                    break;
+                case kStExp1Done:
+                    step = exp_next;
+                    break;
+                case kStExhDone:
+                    step = kStRefineDone;
+                    break;
+                case kStRefineDone: {
+                    // bad-block expansion at coarse levels (:607-672), around the zero vector
+                    found = best.sad;
+                    const int bad_limit = sat_add_i32(g.bad_sad, sat_mul_i32(g.bad_sad, bad_count) / 16);
+                    if (level > 0 && idx > 1 && g.bad_sad > 0 && (int)best.sad > bad_limit) {
+                        is_bad = true;
+                        cur = s;  // the block's own predictor
+                        if (g.bad_range < 0) {
+                            ring = g.pel;
+                            step = kStBadRing;
+                        } else if (g.bad_range > 0) {
+                            hex_r = g.bad_range * g.pel;
+                            exp_next = kStBadFinal;
+                            P = Pattern{kPatUmh, 0, 0, hex_r, 0, pat_count(kPatUmh, hex_r, 0)};
+                            step = kStUmhDone;
+                            next = true;
+                        } else {
+                            step = kStBadFinal;
+                        }
+                    } else {
+                        step = kStEnd;
+                    }
+                    break;
+                }
+                case kStBadRing:
+                    if (ring < -g.bad_range * g.pel) {
+                        P = Pattern{kPatExpand, 0, 0, ring, g.pel, pat_count(kPatExpand, ring, g.pel)};
+                        step = kStBadRingDone;
+                        next = true;
+                    } else {
+                        step = kStBadFinal;
+                    }
+                    break;
+                case kStBadRingDone:
+                    if (best.sad < found / 4) {
+                        step = kStBadFinal;
+                    } else {
+                        ring += g.pel;
+                        step = kStBadRing;
+                    }
+                    break;
+                case kStBadFinal:
+                    P = Pattern{kPatExpand, best.mvx, best.mvy, 1, 1, pat_count(kPatExpand, 1, 1)};
+                    step = kStBadFinalDone;
+                    next = true;
+                    break;
+                case kStBadFinalDone:
+                default:
+                    step = kStEnd;
+                    break;
             }
-            expanding(b, e, best.mvx, best.mvy, 1, 1, best);
         }
-        const uint8_t dc = luma_dc(b);
-        if (lane == 0) {
-            DVec o;
-            o.dx = (int16_t)best.mvx;
-            o.dy = (int16_t)best.mvy;
-            o.score = sat_mul(best.sad, smallest ? 3u : 1u);
-            o.luma = dc;
-            o.done = 1;
-            ch.field[idx] = o;
-        }
+        if (!next) break;
+    }
+    *bad = is_bad;
+    *found_out = found;
+    DVec o;
+    o.dx = (int16_t)best.mvx;
+    o.dy = (int16_t)best.mvy;
+    o.score = sat_mul(best.sad, smallest ? 3u : 1u);
+    o.luma = luma_dc(b);
+    o.done = 1;
+    return o;
+}
+
+// exact_search_level (:480-681) in the reference's visit order: one wavefront per
+// chain.  Used where the parallel sweeps do not apply (the coarse-level bad-block
+// pass keeps a running count over the visit order) and as their fallback.
+__global__ void __launch_bounds__(kWave, 3) search_level_kernel(const Chain* __restrict__ chains) {
+    const Chain ch = uniform_load(&chains[blockIdx.x]);
+    const LevelGeom g = uniform_load(ch.geom);
+    Blk b;
+    init_blk(ch, g, b);
+    int bad_count = 0;
+    const int nblocks = g.gw * g.gh;
+    for (int it = 0; it < nblocks; it++) {
+        const int packed = uniform_int(ch.order[it]);
+        const int bx = packed & 0xFFFF, by = packed >> 16;
+        if (bx >= g.gw || by >= g.gh) continue;  // defensive: orders always hold in-grid positions
+        const int idx = by * g.gw + bx;
+        bool bad;
+        uint32_t found;
+        const DVec o = search_block(ch, g, b, idx, bad_count, &bad, &found);
+        bad_count += bad ? 1 : 0;
+        if (threadIdx.x == 0) ch.res[idx] = o;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    }
+}
+
+// Per level: visit positions from the order, results initialised to the predictors,
+// sweep flags cleared, sweep counters zeroed.
+__global__ void sweep_init_kernel(const Chain* __restrict__ chains, SweepState st) {
+    const Chain ch = chains[blockIdx.y];
+    const LevelGeom g = *ch.geom;
+    const int n = g.gw * g.gh;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int packed = ch.order[i];
+        const int bx = packed & 0xFFFF, by = packed >> 16;
+        if (bx < g.gw && by < g.gh) ch.opos[by * g.gw + bx] = i;
+        ch.res[i] = ch.in[i];
+        st.flags[0][ch.base + i] = 0;
+        st.flags[1][ch.base + i] = 0;
+        st.found[ch.base + i] = 0;
+        st.badf[ch.base + i] = 0;
+        st.cpre[ch.base + i] = 1 << 30;  // sweep 0 runs no bad-block pass: the limit saturates
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) st.minpos[blockIdx.y] = 0;  // the first rebuild covers the whole order
+    if (blockIdx.x == 0 && blockIdx.y == 0)
+        for (int k = threadIdx.x; k < st.max_sweeps + 2; k += blockDim.x) st.counts[k] = 0;
+}
+
+__device__ __forceinline__ void queue_block(const SweepState& st, int ci, const Chain& ch, int n, int s) {
+    const int q = (s + 1) & 1;
+    if (atomicExch(&st.flags[q][ch.base + n], 1u) == 0u) {
+        const uint32_t pos = atomicAdd(&st.counts[s + 1], 1u);
+        st.wl[q][pos] = ((unsigned long long)ci << 32) | (unsigned)n;
+    }
+}
+
+// After sweep s, for chains with the coarse bad-block pass: rebuild the running bad
+// count over the visit order from the first position whose found/bad changed, and
+// queue every block whose bad flag no longer matches its count.  One wave per chain;
+// the recurrence (the limit grows with the count) runs over 64 positions at a time.
+__global__ void __launch_bounds__(kWave) sweep_scan_kernel(const Chain* __restrict__ chains, SweepState st, int s) {
+    const int ci = blockIdx.x;
+    const Chain ch = chains[ci];
+    const LevelGeom g = *ch.geom;
+    if (g.level == 0 || g.bad_sad <= 0) return;
+    const uint32_t start = st.minpos[ci];
+    if (start == 0xFFFFFFFFu) return;
+    const int n = g.gw * g.gh;
+    const int lane = threadIdx.x;
+    int c = 0;
+    if (start > 0) {
+        const int p0 = ch.order[start];
+        c = st.cpre[ch.base + (p0 >> 16) * g.gw + (p0 & 0xFFFF)];
+    }
+    for (int base = (int)start; base < n; base += kWave) {
+        const int k = base + lane;
+        int idx = -1;
+        uint32_t f = 0;
+        if (k < n) {
+            const int pk = ch.order[k];
+            idx = (pk >> 16) * g.gw + (pk & 0xFFFF);
+            f = st.found[ch.base + idx];
+        }
+        int myc = 0, mybad = 0;
+        const int cnt = min(kWave, n - base);
+        for (int j = 0; j < cnt; j++) {
+            const int ij = __shfl(idx, j);
+            const uint32_t fj = __shfl(f, j);
+            const int lim = sat_add_i32(g.bad_sad, sat_mul_i32(g.bad_sad, c) / 16);
+            const int bj = (ij > 1 && (int)fj > lim) ? 1 : 0;
+            if (lane == j) { myc = c; mybad = bj; }
+            c += bj;
+        }
+        if (k < n) {
+            st.cpre[ch.base + idx] = myc;
+            if ((uint32_t)mybad != st.badf[ch.base + idx]) queue_block(st, ci, ch, idx, s);
+        }
+    }
+    if (lane == 0) st.minpos[ci] = 0xFFFFFFFFu;
+}
+
+// Evaluates block idx of chain ci in sweep s; when its vector (or score) changes, the
+// result is stored and every 3x3 neighbour visited after it is queued for sweep s + 1.
+__device__ __forceinline__ void sweep_item(const Chain* __restrict__ chains, int ci, int idx, const SweepState& st,
+                                           int s) {
+    const Chain ch = uniform_load(&chains[ci]);
+    const LevelGeom g = uniform_load(ch.geom);
+    Blk b;
+    init_blk(ch, g, b);
+    bool bad;
+    uint32_t found;
+    const DVec o = search_block(ch, g, b, idx, uniform_int(st.cpre[ch.base + idx]), &bad, &found);
+    const DVec old = uniform_load(&ch.res[idx]);
+    const bool moved = o.dx != old.dx || o.dy != old.dy || o.score != old.score;
+    const uint32_t oldf = (uint32_t)uniform_int((int)st.found[ch.base + idx]);
+    const uint32_t oldb = (uint32_t)uniform_int((int)st.badf[ch.base + idx]);
+    if (threadIdx.x == 0) {
+        if (moved || o.luma != old.luma || o.done != old.done) ch.res[idx] = o;
+        if (found != oldf || (uint32_t)bad != oldb) {
+            st.found[ch.base + idx] = found;
+            st.badf[ch.base + idx] = bad ? 1u : 0u;
+            if (g.level > 0 && g.bad_sad > 0) atomicMin(&st.minpos[ci], (uint32_t)ch.opos[idx]);
+        }
+    }
+    if (!moved) return;
+    const int lane = threadIdx.x;
+    if (lane < 9 && lane != 4) {
+        const int bx = idx % g.gw, by = idx / g.gw;
+        const int nx = bx - 1 + lane % 3, ny = by - 1 + lane / 3;
+        if (nx >= 0 && ny >= 0 && nx < g.gw && ny < g.gh) {
+            const int n = ny * g.gw + nx;
+            if (ch.opos[n] > ch.opos[idx]) queue_block(st, ci, ch, n, s);
+        }
+    }
+}
+
+// Sweep 0: every block of every chain (grid: blocks x chains).
+__global__ void __launch_bounds__(kWave, 3) sweep0_kernel(const Chain* __restrict__ chains, SweepState st) {
+    const int ci = blockIdx.y, idx = blockIdx.x;
+    const LevelGeom* g = chains[ci].geom;
+    if (idx >= g->gw * g->gh) return;
+    sweep_item(chains, ci, idx, st, 0);
+}
+
+// Sweep s >= 1 over the blocks queued by sweep s - 1.
+__global__ void __launch_bounds__(kWave, 3) sweep_kernel(const Chain* __restrict__ chains, SweepState st, int s) {
+    const uint32_t n = (uint32_t)uniform_int((int)__atomic_load_n(&st.counts[s], __ATOMIC_RELAXED));
+    const int p = s & 1;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const unsigned long long item = st.wl[p][i];
+        // uniform across the wave: keeps the chain's descriptors in scalar registers
+        const int ci = __builtin_amdgcn_readfirstlane((int)(item >> 32));
+        const int idx = __builtin_amdgcn_readfirstlane((int)(item & 0xFFFFFFFFu));
+        if (threadIdx.x == 0) st.flags[p][chains[ci].base + idx] = 0;
+        sweep_item(chains, ci, idx, st, s);
     }
 }
 
@@ -1064,6 +1223,26 @@ __global__ void pack_kernel(const PackJob* __restrict__ jobs, int denom) {
 
 hipError_t launch_search_level(const Chain* d_chains, int nchains, hipStream_t s) {
     hipLaunchKernelGGL(search_level_kernel, dim3(nchains), dim3(kWave), 0, s, d_chains);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_init(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s) {
+    hipLaunchKernelGGL(sweep_init_kernel, dim3((max_n + 255) / 256, nchains), dim3(256), 0, s, d_chains, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s) {
+    hipLaunchKernelGGL(sweep0_kernel, dim3(max_n, nchains), dim3(kWave), 0, s, d_chains, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s) {
+    hipLaunchKernelGGL(sweep_kernel, dim3(grid), dim3(kWave), 0, s, d_chains, st, sweep);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_scan(const Chain* d_chains, int nchains, const SweepState& st, int sweep, hipStream_t s) {
+    hipLaunchKernelGGL(sweep_scan_kernel, dim3(nchains), dim3(kWave), 0, s, d_chains, st, sweep);
     return hipGetLastError();
 }
 

@@ -5,13 +5,12 @@
 
 namespace svpf {
 
-// One block vector of a level field (analyse_search.rs:25-31 Vec8 plus a
-// "searched" flag that replaces the reference's `processed` grid).  16 bytes.
+// One block vector of a level field (analyse_search.rs:25-31 Vec8), padded to 16 bytes.
 struct alignas(16) DVec {
     int16_t dx, dy;
     uint32_t score;
     uint32_t luma;
-    uint32_t done;
+    uint32_t done;  // unused by the search (visit positions decide which neighbours seed)
 };
 
 // Planes of one super frame (device pointers, byte strides, byte lengths).
@@ -47,11 +46,32 @@ struct LevelGeom {
 struct Chain {
     SPlanes src;           // frame whose blocks are searched
     SPlanes ref;           // reference frame
-    DVec* field;           // in: interpolated predictors; out: searched vectors
+    const DVec* in;        // interpolated predictors of the level (read only)
+    DVec* res;             // searched vectors (the level's result)
     const DVec* prev;      // projected opposite-direction field (nullptr at the top level)
     const int32_t* order;  // visit order, packed x | y << 16
+    int32_t* opos;         // position of each block in the visit order (raster indexed)
     const LevelGeom* geom;
     const int32_t* global; // (gx, gy) global predictor from the coarser level; nullptr = (0, 0)
+    uint32_t base;         // first index of this chain's blocks in the level's flag arrays
+};
+
+// Worklists of the parallel sweeps of one level (all chains together).  Sweep s
+// processes wl[s & 1][0 .. counts[s]) whose members are flagged in flags[s & 1]; a
+// block whose result changes queues its later-visited 3x3 neighbours for sweep s + 1.
+// Coarse levels with the bad-block pass also keep, per block, the pre-expansion cost
+// (`found`), whether the expansion ran (`badf`) and the running count of bad blocks
+// visited before it (`cpre`, rebuilt after every sweep from the first visit position
+// whose found/bad changed, `minpos` per chain).
+struct SweepState {
+    unsigned long long* wl[2];  // items: chain index << 32 | block raster index
+    uint32_t* flags[2];         // per block (chain base + raster index): queued for that parity
+    uint32_t* counts;           // counts[s] = items of sweep s (counts[0] unused)
+    uint32_t* found;            // per block
+    uint32_t* badf;             // per block
+    int32_t* cpre;              // per block
+    uint32_t* minpos;           // per chain
+    int max_sweeps;
 };
 
 

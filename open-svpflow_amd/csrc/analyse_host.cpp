@@ -11,6 +11,10 @@
 namespace svpf {
 
 hipError_t launch_search_level(const Chain* d_chains, int nchains, hipStream_t s);
+hipError_t launch_sweep_init(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s);
+hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s);
+hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s);
+hipError_t launch_sweep_scan(const Chain* d_chains, int nchains, const SweepState& st, int sweep, hipStream_t s);
 hipError_t launch_interp(const void* d_tr, int ntr, int max_n, hipStream_t s);
 hipError_t launch_global(const void* d_jobs, int njobs, hipStream_t s);
 hipError_t launch_sort_keys(const void* d_jobs, int njobs, int max_n, hipStream_t s);
@@ -140,6 +144,36 @@ struct ClipPlan {
     size_t header_off;     // 16 header words (blob)
 };
 
+constexpr int kMaxSweeps = 4096;
+constexpr int kNotConverged = 1;
+
+// Parallel sweeps of one level until no block changes (each sweep evaluates the blocks
+// whose earlier-visited neighbours changed in the previous sweep).  The fixed point is
+// unique -- every block depends only on blocks visited before it -- and equals the
+// sequential result.  The host checks the queue length after each chunk of sweeps;
+// sweeps launched past convergence find an empty queue and return at once.
+int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total_blocks, const SweepState& st,
+               bool scan) {
+    if (!ctx->h_count) SVPF_HIP(hipHostMalloc((void**)&ctx->h_count, 64, hipHostMallocDefault));
+    SVPF_HIP(launch_sweep0(dch, nch, max_n, st, ctx->stream));
+    if (scan) SVPF_HIP(launch_sweep_scan(dch, nch, st, 0, ctx->stream));
+    const int grid = (int)std::min<size_t>(std::max<size_t>(total_blocks, 1), 4096);
+    int s = 1, chunk = 8;
+    for (;;) {
+        if (s + chunk > st.max_sweeps) return kNotConverged;
+        for (int k = 0; k < chunk; k++) {
+            SVPF_HIP(launch_sweep(dch, grid, st, s + k, ctx->stream));
+            if (scan) SVPF_HIP(launch_sweep_scan(dch, nch, st, s + k, ctx->stream));
+        }
+        s += chunk;
+        SVPF_HIP(hipMemcpyAsync(ctx->h_count, st.counts + s, 4, hipMemcpyDeviceToHost, ctx->stream));
+        SVPF_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->sweeps_last += chunk;
+        if (*ctx->h_count == 0) return SVPF_OK;
+        chunk = std::min(chunk * 2, 64);
+    }
+}
+
 }  // namespace
 
 int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, const svpf_frame* d_cur,
@@ -244,7 +278,8 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
     // ---- device workspace layout (per chain per level buffers)
     Carve work;
     struct ChainBufs {
-        size_t field[16], proj[16], key[16], order[16], glob[16];
+        size_t field[16], res[16], opos[16], proj[16], key[16], order[16], glob[16];
+        uint32_t base[16];  // first index in the level's sweep flag arrays
     };
     std::vector<ChainBufs> cb(nchains);
     size_t sort_total[16] = {};
@@ -254,11 +289,38 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
         for (int l = 0; l < c.levels; l++) {
             size_t cnt = (size_t)c.gw[l] * c.gh[l];
             cb[ci].field[l] = work.take(sizeof(DVec) * cnt);
+            cb[ci].res[l] = work.take(sizeof(DVec) * cnt);
+            cb[ci].opos[l] = work.take(4 * cnt);
             cb[ci].proj[l] = work.take(sizeof(DVec) * cnt);
             cb[ci].key[l] = work.take(8 * cnt);
             cb[ci].order[l] = work.take(4 * cnt);
             cb[ci].glob[l] = work.take(8);
         }
+    }
+    // sweep state per level: flags x2 (one entry per block of every chain), worklists x2, counters
+    size_t level_blocks[16] = {}, flags_off[16][2], wl_off[16][2], counts_off[16], found_off[16], badf_off[16],
+        cpre_off[16], minpos_off[16];
+    int level_max_n[16] = {};
+    for (int l = 0; l < max_levels; l++) {
+        size_t tot = 0;
+        for (int ci = 0; ci < nchains; ci++) {
+            const ClipPlan& c = clips[clip_of[ci / 2]];
+            if (l >= c.levels) continue;
+            cb[ci].base[l] = (uint32_t)tot;
+            tot += (size_t)c.gw[l] * c.gh[l];
+            level_max_n[l] = std::max(level_max_n[l], c.gw[l] * c.gh[l]);
+        }
+        if (tot >= 0xFFFFFFFFull) return fail(SVPF_ERR_ARG, "svpf_analyse: batch too large");
+        level_blocks[l] = tot;
+        for (int q = 0; q < 2; q++) {
+            flags_off[l][q] = work.take(4 * tot + 4);
+            wl_off[l][q] = work.take(8 * tot + 8);
+        }
+        counts_off[l] = work.take(4 * (size_t)(kMaxSweeps + 2));
+        found_off[l] = work.take(4 * tot + 4);
+        badf_off[l] = work.take(4 * tot + 4);
+        cpre_off[l] = work.take(4 * tot + 4);
+        minpos_off[l] = work.take(4 * (size_t)nchains + 4);
     }
     for (int l = 0; l < max_levels; l++) {
         size_t tot = 0;
@@ -328,15 +390,19 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
         const svpf_analyse_params& p = *clips[clip_of[ci / 2]].p;
         return (ci & 1) ? (p.vectors & 1) != 0 : (p.vectors & 2) != 0;  // even = fwd, odd = bwd
     };
+    bool too_large = false;
     auto to_splanes = [&](const svpf_frame& f, const svpf_analyse_params& p) {
         SPlanes s;
         int yh = super_plane_height(p.height, p.pel, p.super_levels, true), uvh = yh / 2;
         s.p[0] = f.y; s.p[1] = f.u; s.p[2] = f.v;
         s.stride[0] = f.stride_y; s.stride[1] = f.stride_u; s.stride[2] = f.stride_v;
         s.len[0] = (int64_t)f.stride_y * yh; s.len[1] = (int64_t)f.stride_u * uvh; s.len[2] = (int64_t)f.stride_v * uvh;
+        for (int k = 0; k < 3; k++)
+            if (s.len[k] >= INT32_MAX) too_large = true;  // the kernels index planes with 32-bit offsets
         return s;
     };
     int nch_step[16] = {}, ntr_step[16] = {}, nsort_step[16] = {}, max_tr_n[16] = {}, max_sort_n[16] = {};
+    bool seq_step[16] = {}, bad_step[16] = {};  // sequential fallback / bad-block count rebuilds
     Transition* inits = (Transition*)&host[init_off];
     int ninit = 0, max_init_n = 0;
     for (int step = max_levels - 1; step >= 0; step--) {
@@ -362,7 +428,10 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
             // fwd searches cur against ref; bwd searches ref against cur (analyse_search.rs:160-191)
             ch.src = to_splanes((ci & 1) ? d_ref[pi] : d_cur[pi], p);
             ch.ref = to_splanes((ci & 1) ? d_cur[pi] : d_ref[pi], p);
-            ch.field = (DVec*)(W + cb[ci].field[l]);
+            ch.in = (const DVec*)(W + cb[ci].field[l]);
+            ch.res = (DVec*)(W + cb[ci].res[l]);
+            ch.opos = (int32_t*)(W + cb[ci].opos[l]);
+            ch.base = cb[ci].base[l];
             ch.prev = top ? nullptr : (const DVec*)(W + cb[partner].proj[l]);
             if (top) ch.order = (const int32_t*)blob_ptr(c.order_top_off);
             else if (l == 0) ch.order = (const int32_t*)blob_ptr(c.order0_off);
@@ -385,7 +454,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
             if (!top) {
                 // transition from level l+1 into level l
                 Transition t{};
-                t.coarse = (const DVec*)(W + cb[ci].field[l + 1]);
+                t.coarse = (const DVec*)(W + cb[ci].res[l + 1]);
                 t.next = (DVec*)(W + cb[ci].field[l]);
                 t.proj_key = (unsigned long long*)(W + cb[ci].key[l]);
                 t.proj = (DVec*)(W + cb[ci].proj[l]);
@@ -430,7 +499,10 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
         nch_step[l] = nc;
         ntr_step[l] = nt;
         nsort_step[l] = ns;
+        for (auto& c : clips)
+            if (l > 0 && l < c.levels && c.geom[l].bad_sad > 0) bad_step[l] = true;
     }
+    if (too_large) return fail(SVPF_ERR_ARG, "svpf_analyse: super-frame plane of 2 GiB or more");
     PackJob* pj = (PackJob*)&host[pack_off];
     int npack = 0, max_count = 0;
     for (int ci = 0; ci < nchains; ci++) {
@@ -448,7 +520,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
         const size_t region = 4 + (size_t)count * 8;
         const size_t off = 0x40 + ((!is_bwd && (p.vectors & 1)) ? region : 0);
         PackJob j{};
-        j.field = (const DVec*)(W + cb[ci].field[0]);
+        j.field = (const DVec*)(W + cb[ci].res[0]);
         j.out = d_payload[pi] + off;
         j.payload = d_payload[pi];
         j.header = (const int32_t*)blob_ptr(c.header_off);
@@ -497,7 +569,25 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
         }
         if (nch_step[l]) {
             const int ph = prof_begin(ctx, kProfSearch);
-            SVPF_HIP(launch_search_level((const Chain*)blob_ptr(chain_off[l]), nch_step[l], ctx->stream));
+            const Chain* dch = (const Chain*)blob_ptr(chain_off[l]);
+            SweepState st{};
+            for (int q = 0; q < 2; q++) {
+                st.wl[q] = (unsigned long long*)(W + wl_off[l][q]);
+                st.flags[q] = (uint32_t*)(W + flags_off[l][q]);
+            }
+            st.counts = (uint32_t*)(W + counts_off[l]);
+            st.found = (uint32_t*)(W + found_off[l]);
+            st.badf = (uint32_t*)(W + badf_off[l]);
+            st.cpre = (int32_t*)(W + cpre_off[l]);
+            st.minpos = (uint32_t*)(W + minpos_off[l]);
+            st.max_sweeps = kMaxSweeps;
+            SVPF_HIP(launch_sweep_init(dch, nch_step[l], level_max_n[l], st, ctx->stream));
+            int rc2 = seq_step[l] ? SVPF_OK
+                                  : run_sweeps(ctx, dch, nch_step[l], level_max_n[l], level_blocks[l], st, bad_step[l]);
+            if (rc2 == kNotConverged || seq_step[l])
+                SVPF_HIP(launch_search_level(dch, nch_step[l], ctx->stream));
+            else if (rc2)
+                return rc2;
             prof_end(ctx, ph);
         }
     }

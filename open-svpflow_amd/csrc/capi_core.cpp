@@ -166,6 +166,7 @@ void svpf_destroy(svpf_ctx* ctx) {
     if (ctx->h_desc) (void)hipHostFree(ctx->h_desc);
     prof_clear(ctx);
     for (auto e : ctx->prof_pool) (void)hipEventDestroy(e);
+    if (ctx->h_count) (void)hipHostFree(ctx->h_count);
     (void)hipEventDestroy(ctx->desc_done);
     (void)hipEventDestroy(ctx->t0);
     (void)hipEventDestroy(ctx->t1);

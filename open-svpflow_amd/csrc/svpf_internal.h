@@ -24,6 +24,8 @@ struct svpf_ctx {
     size_t work_cap = 0;
     void* d_sort_tmp = nullptr;
     size_t sort_tmp_cap = 0;
+    uint32_t* h_count = nullptr;  // pinned word for reading back sweep queue lengths
+    long long sweeps_last = 0;    // sweeps launched by Analyse calls (statistics)
     // Optional kernel timing: HIP event pairs recorded around launches on `stream`.
     struct ProfMark {
         int kernel;
