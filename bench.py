@@ -173,9 +173,6 @@ def main():
     for _ in range(args.warmup):
         clip.batched_step()
     ctx.sync()
-    dec = clip.smooth.render(0, clip.out, decisions=True)
-    rendered = sum(1 for d in dec if d[0] == 2)
-    copies = len(dec) - rendered
 
     print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
     ctx.profile(True)
@@ -188,7 +185,11 @@ def main():
     ctx.sync()
     wall = time.perf_counter() - t0
     prof = {k: ctx.profile_read(k) for k in ("super", "search", "smooth_prep", "render")}
+    sweeps = {lv: ctx.analyse_stats(lv) for lv in range(clip.ap.levels)}
     ctx.profile(False)
+    dec = clip.smooth.render(0, clip.out, decisions=True)
+    rendered = sum(1 for d in dec if d[0] == 2)
+    copies = len(dec) - rendered
     ms_per_step = 1000.0 * wall / args.steps
     value = clip.nout * args.steps / wall
 
@@ -253,6 +254,7 @@ def main():
                            "frac": super_gbs / HBM_PEAK_GBS, "launches": super_n,
                            "avg_launch_ms": round(super_ms / max(super_n, 1), 3)},
         "stage_ms_per_step": {k: round(v[0] / args.steps, 2) for k, v in prof.items()},
+        "analyse_sweeps": {f"level{lv}": v for lv, v in sweeps.items()},
         "p99_frame_latency_ms": None,
         "cpu_baseline": cpu,
     }

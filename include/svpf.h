@@ -66,6 +66,10 @@ int svpf_timer_stop(svpf_ctx* ctx, float* ms);
  * the summed milliseconds and the number of timed launches of one kernel. */
 int svpf_profile_enable(svpf_ctx* ctx, int on);
 int svpf_profile_read(svpf_ctx* ctx, const char* kernel, double* total_ms, int32_t* launches);
+/* Analyse sweep statistics of one pyramid level since profiling was enabled (no
+ * counterpart in the reference): out4 = {level runs, sweeps launched, sweeps that had
+ * work, block evaluations}. */
+int svpf_analyse_stats(svpf_ctx* ctx, int32_t level, int64_t* out4);
 
 /* ------------------------------------------------------------------ Super */
 /* Replaces SuperOpts (crates/svpflow1/src/super_opts.rs:3-95). */

@@ -27,6 +27,45 @@ namespace {
 constexpr int kWave = 64;
 using namespace cost;
 
+// Lane of the calling thread within its wavefront (kernels here run one or more whole
+// wavefronts per workgroup).
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
+
+// Packed 16-bit SATD of one 4x4 tile (same value as had4_rows): the differences and
+// the first three butterfly stages run two columns per instruction; the last stage
+// uses |a + b| + |a - b| = 2 max(|a|, |b|), so it needs no butterflies.  Every
+// intermediate stays within +-2040.
+typedef short __attribute__((ext_vector_type(2))) s16x2;
+
+__device__ __forceinline__ s16x2 as_s16x2(uint32_t v) { return __builtin_bit_cast(s16x2, v); }
+__device__ __forceinline__ uint32_t pk_bytes01(uint32_t v) { return __builtin_amdgcn_perm(0u, v, 0x0c010c00u); }
+__device__ __forceinline__ uint32_t pk_bytes23(uint32_t v) { return __builtin_amdgcn_perm(0u, v, 0x0c030c02u); }
+
+__device__ __forceinline__ uint32_t had4_rows_pk(const uint32_t a[4], const uint32_t b[4]) {
+    s16x2 P[4], M[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const s16x2 A = as_s16x2(pk_bytes01(a[r])) - as_s16x2(pk_bytes01(b[r]));  // columns 0, 1
+        const s16x2 B = as_s16x2(pk_bytes23(a[r])) - as_s16x2(pk_bytes23(b[r]));  // columns 2, 3
+        P[r] = A + B;
+        M[r] = A - B;
+    }
+    s16x2 q[8];
+    {
+        const s16x2 p01s = P[0] + P[1], p01d = P[0] - P[1], p23s = P[2] + P[3], p23d = P[2] - P[3];
+        const s16x2 m01s = M[0] + M[1], m01d = M[0] - M[1], m23s = M[2] + M[3], m23d = M[2] - M[3];
+        q[0] = p01s + p23s; q[1] = p01s - p23s; q[2] = p01d + p23d; q[3] = p01d - p23d;
+        q[4] = m01s + m23s; q[5] = m01s - m23s; q[6] = m01d + m23d; q[7] = m01d - m23d;
+    }
+    s16x2 acc = {0, 0};
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const s16x2 v = __builtin_elementwise_max(q[k], -q[k]);  // |.|
+        acc += __builtin_elementwise_max(v, v.yx);                // max of the pair, in both halves
+    }
+    return 2u * (uint32_t)(uint16_t)acc.x;
+}
+
 // Interior block cost: every source and reference byte lies inside its level, so rows
 // are read as 32-bit words (unaligned) with no per-byte checks.  N x M block.
 template <int N, int M>
@@ -85,12 +124,120 @@ __device__ __forceinline__ uint32_t xor_sum8(uint32_t v) {
     return v;
 }
 
+// ---- LDS copy of one block's source and reference search window (one region per wave)
+
+typedef __attribute__((address_space(3))) uint8_t lds_u8;
+typedef __attribute__((address_space(3))) uint32_t __attribute__((aligned(1))) lds_u32u;
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+constexpr int kWinBytes = 12288;  // reference window budget per wave
+constexpr int kSrcBytes = 1536;   // source block: luma + 2 chroma, up to 32x32
+constexpr int kLdsPerWave = kWinBytes + kSrcBytes;
+
+__device__ __forceinline__ uint32_t lds_ld32(const lds_u8* p) { return *reinterpret_cast<const lds_u32u*>(p); }
+
+// The staged window (wave-uniform).  Candidates whose interior luma and chroma blocks
+// lie inside it read LDS; any other candidate reads the planes as before.
+struct Win {
+    int on;
+    int lx0, ly0, lw, lh, lpitch;  // luma: level pixels [lx0, lx0 + lw) x [ly0, ly0 + lh) per slot
+    int cx0, cy0, cw, ch, cpitch;  // chroma
+    int ref_u, ref_v;              // LDS offsets of the chroma windows (luma window at 0)
+    int src_l, src_u, src_v;       // LDS offsets of the source blocks (pitch bw / cw)
+};
+
+// Copies rows [y0, y0 + h) x columns [x0, x0 + w) of one plane region (level pixels,
+// first plane row `row0`, level size lw x lh) into LDS at `dst` with `pitch`; pixels
+// outside the level are written as 0 (no candidate that reads LDS uses them).
+__device__ __forceinline__ void stage_rows(lds_u8* dst, int pitch, const uint8_t* plane, int plen, int stride, int row0,
+                                           int x0, int y0, int w, int h, int lw, int lh) {
+    const int wd = pitch >> 2;
+    for (int e = lane_id(); e < h * wd; e += kWave) {
+        const int r = e / wd, c = (e - r * wd) * 4;
+        const int gy = y0 + r, gx = x0 + c;
+        uint32_t v = 0;
+        if (gy >= 0 && gy < lh) {
+            const int base = (row0 + gy) * stride;
+            if (gx >= 0 && gx + 4 <= lw) {
+                v = ld32(plane + base + gx);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int x = gx + k;
+                    if (x >= 0 && x < lw) v |= (uint32_t)rdb(plane, plen, base + x, 0) << (8 * k);
+                }
+            }
+        }
+        *reinterpret_cast<lds_u32*>(dst + r * pitch + c) = v;
+    }
+}
+
+// Stages the block's source and the reference window covering the vectors within
+// +-V of (cx, cy) (vector units), shrinking V to the LDS budget.  Wave-cooperative.
+__device__ __forceinline__ void stage_window(const Blk& b, lds_u8* lds, int cx, int cy, int V, bool chroma, Win& w) {
+    w.on = 0;
+    if (!lds || !pow2_4_32(b.bw) || b.bw != b.bh) return;
+    const bool subpel = b.level == 0 && b.pel >= 2;
+    const int sh = subpel ? 1 : 0, csh = subpel ? 2 : 1;
+    const int nsub = subpel ? 4 : 1;
+    const int ipx = b.px - ((b.level > 0 && b.px + b.bw >= b.lw) ? 1 : 0);
+    const int ipy = b.py - ((b.level > 0 && b.py + b.bh >= b.lh) ? 1 : 0);
+    const int cwb = b.bw / 2, chb = b.bh / 2;
+    const bool cuse = chroma && cwb >= 4 && chb >= 4;
+    const int cpx = b.px / 2, cpy = b.py / 2;
+    for (; V >= 2; V >>= 1) {
+        w.lx0 = ipx + ((cx - V) >> sh);
+        w.ly0 = ipy + ((cy - V) >> sh);
+        w.lw = b.bw + ((cx + V) >> sh) - ((cx - V) >> sh);
+        w.lh = b.bh + ((cy + V) >> sh) - ((cy - V) >> sh);
+        w.lpitch = (w.lw + 3) & ~3;
+        w.cx0 = cpx + ((cx - V) >> csh);
+        w.cy0 = cpy + ((cy - V) >> csh);
+        w.cw = cwb + ((cx + V) >> csh) - ((cx - V) >> csh);
+        w.ch = chb + ((cy + V) >> csh) - ((cy - V) >> csh);
+        w.cpitch = (w.cw + 3) & ~3;
+        const int lbytes = nsub * w.lh * w.lpitch;
+        const int cbytes = cuse ? nsub * w.ch * w.cpitch : 0;
+        if (lbytes + 2 * cbytes <= kWinBytes) {
+            w.ref_u = lbytes;
+            w.ref_v = lbytes + cbytes;
+            w.on = 1;
+            break;
+        }
+    }
+    if (!w.on) return;
+    w.src_l = kWinBytes;
+    w.src_u = kWinBytes + b.bw * b.bh;
+    w.src_v = w.src_u + cwb * chb;
+    for (int k = 0; k < nsub; k++) {
+        const int sub = (k >> 1) * b.pel + (k & 1);
+        stage_rows(lds + k * w.lh * w.lpitch, w.lpitch, b.rp[0], b.rl[0], b.rs[0], b.y_off + sub * b.lh, w.lx0, w.ly0,
+                   w.lw, w.lh, b.lw, b.lh);
+        if (cuse) {
+            const int crow = b.c_off + (subpel ? sub * b.clh : 0);
+            stage_rows(lds + w.ref_u + k * w.ch * w.cpitch, w.cpitch, b.rp[1], b.rl[1], b.rs[1], crow, w.cx0, w.cy0, w.cw,
+                       w.ch, b.clw, b.clh);
+            stage_rows(lds + w.ref_v + k * w.ch * w.cpitch, w.cpitch, b.rp[2], b.rl[2], b.rs[2], crow, w.cx0, w.cy0, w.cw,
+                       w.ch, b.clw, b.clh);
+        }
+    }
+    stage_rows(lds + w.src_l, b.bw, b.sp[0], b.sl[0], b.ss[0], b.y_off, ipx, ipy, b.bw, b.bh, b.lw, b.lh);
+    if (cuse) {
+        stage_rows(lds + w.src_u, cwb, b.sp[1], b.sl[1], b.ss[1], b.c_off, cpx, cpy, cwb, chb, b.clw, b.clh);
+        stage_rows(lds + w.src_v, cwb, b.sp[2], b.sl[2], b.ss[2], b.c_off, cpx, cpy, cwb, chb, b.clw, b.clh);
+    }
+    // the lanes read what other lanes of the wave wrote
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // One candidate's cost computed by the 8 lanes of a group (part = lane & 7); every
 // lane of the group receives (L, C).  Blocks whose luma and chroma lie inside their
 // levels (the common case) split into units -- 4x4 tiles for SATD, rows for SAD --
 // spread over the group; any other path is computed whole by part 0 (cand_cost_full).
-__device__ __forceinline__ void group_cost(const Blk& b, int mvx, int mvy, bool chroma, int part, uint32_t& Lout,
-                                           uint32_t& Cout) {
+__device__ __forceinline__ void group_cost(const Blk& b, const lds_u8* lds, const Win& win, int mvx, int mvy,
+                                           bool chroma, int part, uint32_t& Lout, uint32_t& Cout) {
     const int pel = b.pel;
     const int b0 = -pel * b.px, b1 = -pel * b.py;
     const int b2 = pel * (b.lw - b.px - b.bw) + 1, b3 = pel * (b.lh - b.py - b.bh) + 1;
@@ -114,8 +261,58 @@ __device__ __forceinline__ void group_cost(const Blk& b, int mvx, int mvy, bool 
     const bool cfast = cw == ch && (cw == 4 || cw == 8 || cw == 16) && cpx + cw <= b.clw && cpy + ch <= b.clh &&
                        crx >= 0 && cry >= 0 && crx + cw <= b.clw && cry + ch <= b.clh;
     const bool fast = lfast && (!cuse || cfast);
+    const bool inwin = fast && win.on && frx >= win.lx0 && frx + b.bw <= win.lx0 + win.lw && fry >= win.ly0 &&
+                       fry + b.bh <= win.ly0 + win.lh &&
+                       (!cuse || (crx >= win.cx0 && crx + cw <= win.cx0 + win.cw && cry >= win.cy0 &&
+                                  cry + ch <= win.cy0 + win.ch));
     uint32_t a0 = 0, a1 = 0, a2 = 0;  // luma, u, v partial sums (fast) or (L, C, 0) from part 0
-    if (fast) {
+    if (inwin) {
+        // the same units as below, read from the staged copies
+        const int slot = subpel ? ((mvy & 1) * 2 + (mvx & 1)) : 0;
+        const lds_u8* ls = lds + win.src_l;
+        const lds_u8* lr = lds + (slot * win.lh + (fry - win.ly0)) * win.lpitch + (frx - win.lx0);
+        const lds_u8* us = lds + win.src_u;
+        const lds_u8* ur = lds + win.ref_u + (slot * win.ch + (cry - win.cy0)) * win.cpitch + (crx - win.cx0);
+        const lds_u8* vs = lds + win.src_v;
+        const lds_u8* vr = lds + win.ref_v + (slot * win.ch + (cry - win.cy0)) * win.cpitch + (crx - win.cx0);
+        if (sq) {
+            const int lt = b.bw >> 2, ct = cw >> 2;
+            const int nl = lt * lt, nc = cuse ? ct * ct : 0;
+            for (int u = part; u < nl + 2 * nc; u += kGroup) {
+                const int plane = u < nl ? 0 : (u < nl + nc ? 1 : 2);
+                const int t = plane == 0 ? u : (plane == 1 ? u - nl : u - nl - nc);
+                const int per = plane == 0 ? lt : ct;
+                const int ty = t / per, tx = t - ty * per;
+                const lds_u8* sp = plane == 0 ? ls : (plane == 1 ? us : vs);
+                const lds_u8* rp = plane == 0 ? lr : (plane == 1 ? ur : vr);
+                const int ss = plane == 0 ? b.bw : cw;
+                const int rs = plane == 0 ? win.lpitch : win.cpitch;
+                uint32_t x[4], y[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    x[k] = lds_ld32(sp + (ty * 4 + k) * ss + tx * 4);
+                    y[k] = lds_ld32(rp + (ty * 4 + k) * rs + tx * 4);
+                }
+                const uint32_t h = had4_rows_pk(x, y);
+                if (plane == 0) a0 += h; else if (plane == 1) a1 += h; else a2 += h;
+            }
+        } else {
+            const int nl = b.bh, nc = cuse ? ch : 0;
+            for (int u = part; u < nl + 2 * nc; u += kGroup) {
+                const int plane = u < nl ? 0 : (u < nl + nc ? 1 : 2);
+                const int r = plane == 0 ? u : (plane == 1 ? u - nl : u - nl - nc);
+                const lds_u8* sp = plane == 0 ? ls : (plane == 1 ? us : vs);
+                const lds_u8* rp = plane == 0 ? lr : (plane == 1 ? ur : vr);
+                const int ss = plane == 0 ? b.bw : cw;
+                const int rs = plane == 0 ? win.lpitch : win.cpitch;
+                const int words = (plane == 0 ? b.bw : cw) >> 2;
+                uint32_t acc = 0;
+                for (int w = 0; w < words; w++)
+                    acc = __builtin_amdgcn_sad_u8(lds_ld32(sp + r * ss + 4 * w), lds_ld32(rp + r * rs + 4 * w), acc);
+                if (plane == 0) a0 += acc; else if (plane == 1) a1 += acc; else a2 += acc;
+            }
+        }
+    } else if (fast) {
         const int rrow = b.y_off + sub * b.lh;
         const int cref = b.c_off + ((b.level == 0 && pel > 1) ? sub * b.clh : 0);
         const uint8_t* ls = b.sp[0] + (int64_t)(b.y_off + ipy) * b.ss[0] + ipx;
@@ -142,7 +339,7 @@ __device__ __forceinline__ void group_cost(const Blk& b, int mvx, int mvy, bool 
                     x[k] = ld32(sp + (int64_t)(ty * 4 + k) * ss + tx * 4);
                     y[k] = ld32(rp + (int64_t)(ty * 4 + k) * rs + tx * 4);
                 }
-                const uint32_t h = had4_rows(x, y);
+                const uint32_t h = had4_rows_pk(x, y);
                 if (plane == 0) a0 += h; else if (plane == 1) a1 += h; else a2 += h;
             }
         } else if (!b.satd) {  // SAD rows: luma rows, then u rows, then v rows
@@ -293,7 +490,7 @@ __device__ __attribute__((always_inline)) int adapt_lambda(int nl, int lsad, uin
 
 // Per-lane byte sum helper for block_luma_dc (:1566-1617) and plane activity.
 __device__ __attribute__((always_inline)) uint8_t luma_dc(const Blk& b) {
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     uint32_t sum = 0, cnt = 0;
     // the reference stops a row at the first column past the level, and stops at the first row past it
     const int rows = min(b.bh, max(b.lh - b.py, 0));
@@ -313,7 +510,7 @@ __device__ __attribute__((always_inline)) uint8_t luma_dc(const Blk& b) {
 
 // plane_activity (:1517-1553) of one plane region, wave-parallel.
 __device__ __attribute__((always_inline)) uint32_t plane_activity(const uint8_t* p, int stride, int row, int x, int w, int h, bool satd) {
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     const uint32_t area = (uint32_t)(w * h);
     uint32_t sum = 0;
     for (int i = lane; i < w * h; i += kWave) sum += (uint32_t)rd(p, INT64_MAX, (int64_t)(row + i / w) * stride + x + i % w, 0);
@@ -404,9 +601,9 @@ enum Step : int {
 // predictor -- exactly what the reference's `processed` grid selects in its visit
 // order.  Returns the block's vector (uniform across the wave), *bad = whether the
 // coarse-level bad-block expansion ran and *found_out = the cost before it.
-__device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g, Blk& b, int idx, int bad_count, bool* bad,
-                             uint32_t* found_out) {
-    const int lane = threadIdx.x;
+__device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g, Blk& b, lds_u8* lds, int idx,
+                                             int bad_count, bool* bad, uint32_t* found_out) {
+    const int lane = lane_id();
     const int part = lane & (kGroup - 1);
     const int gx = ch.global ? uniform_int(ch.global[0]) : 0, gy = ch.global ? uniform_int(ch.global[1]) : 0;
     const int gw = g.gw, gh = g.gh, level = g.level;
@@ -498,6 +695,8 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
 
     Best best{0, 0, 0xFFFFFFFFu, INT64_MAX};
     Search cur = s;
+    Win wnd;  // staged search window
+    wnd.on = 0;
     Pattern P;
     bool pchroma;
     int step;
@@ -535,7 +734,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                     valid = in_bounds(cur, mx, my);
                 }
                 uint32_t L = 0, C = 0;
-                if (valid) group_cost(b, mx, my, pchroma, part, L, C);
+                if (valid) group_cost(b, lds, wnd, mx, my, pchroma, part, L, C);
                 const uint32_t sad = sat_add(L, C);
                 int64_t cost = INT64_MAX, motion = INT64_MAX;
                 if (valid) {
@@ -622,6 +821,11 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                     best.cost = (int64_t)best.sad;
                     const int stype = (edge && level == 1) ? 0 : g.search_type;
                     exp_next = kStRefineDone;
+                    if (stype >= 2 && stype <= 4) {
+                        // the refine searches stay near the best predictor: stage its window
+                        const int reach = stype == 4 ? radius : (stype == 3 ? 2 * radius + 4 : radius + 4);
+                        stage_window(b, lds, best.mvx, best.mvy, reach, !smallest, wnd);
+                    }
                     if (stype == 2) {
                         hex_r = radius;
                         step = kStHexBegin;
@@ -774,6 +978,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
 // chain.  Used where the parallel sweeps do not apply (the coarse-level bad-block
 // pass keeps a running count over the visit order) and as their fallback.
 __global__ void __launch_bounds__(kWave, 3) search_level_kernel(const Chain* __restrict__ chains) {
+    __shared__ __attribute__((aligned(16))) uint8_t lds[kLdsPerWave];
     const Chain ch = uniform_load(&chains[blockIdx.x]);
     const LevelGeom g = uniform_load(ch.geom);
     Blk b;
@@ -787,7 +992,7 @@ __global__ void __launch_bounds__(kWave, 3) search_level_kernel(const Chain* __r
         const int idx = by * g.gw + bx;
         bool bad;
         uint32_t found;
-        const DVec o = search_block(ch, g, b, idx, bad_count, &bad, &found);
+        const DVec o = search_block(ch, g, b, (lds_u8*)lds, idx, bad_count, &bad, &found);
         bad_count += bad ? 1 : 0;
         if (threadIdx.x == 0) ch.res[idx] = o;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -811,36 +1016,50 @@ __global__ void sweep_init_kernel(const Chain* __restrict__ chains, SweepState s
         st.badf[ch.base + i] = 0;
         st.cpre[ch.base + i] = 1 << 30;  // sweep 0 runs no bad-block pass: the limit saturates
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) st.minpos[blockIdx.y] = 0;  // the first rebuild covers the whole order
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st.minpos[blockIdx.y] = 0;  // the first rebuild covers the whole order
+        st.tail_count[blockIdx.y] = 0;
+    }
     if (blockIdx.x == 0 && blockIdx.y == 0)
         for (int k = threadIdx.x; k < st.max_sweeps + 2; k += blockDim.x) st.counts[k] = 0;
 }
 
-__device__ __forceinline__ void queue_block(const SweepState& st, int ci, const Chain& ch, int n, int s) {
+// Queue of the next sweep.  Global sweeps (counter == nullptr) append to
+// wl[q][counts[s + 1]++]; the per-chain tail (tail_kernel) appends to the chain's own
+// segment wl[q][base + (*counter)++] with the counter in LDS.
+struct Queue {
+    uint32_t* counter;  // LDS counter of the chain's next list, or nullptr
+};
+
+__device__ __forceinline__ void queue_block(const SweepState& st, const Queue& qu, int ci, const Chain& ch, int n, int s) {
     const int q = (s + 1) & 1;
     if (atomicExch(&st.flags[q][ch.base + n], 1u) == 0u) {
-        const uint32_t pos = atomicAdd(&st.counts[s + 1], 1u);
-        st.wl[q][pos] = ((unsigned long long)ci << 32) | (unsigned)n;
+        if (qu.counter) {
+            const uint32_t pos = atomicAdd(qu.counter, 1u);
+            st.wl[q][ch.base + pos] = ((unsigned long long)ci << 32) | (unsigned)n;
+        } else {
+            const uint32_t pos = atomicAdd(&st.counts[s + 1], 1u);
+            st.wl[q][pos] = ((unsigned long long)ci << 32) | (unsigned)n;
+        }
     }
 }
 
-// After sweep s, for chains with the coarse bad-block pass: rebuild the running bad
-// count over the visit order from the first position whose found/bad changed, and
-// queue every block whose bad flag no longer matches its count.  One wave per chain;
-// the recurrence (the limit grows with the count) runs over 64 positions at a time.
-__global__ void __launch_bounds__(kWave) sweep_scan_kernel(const Chain* __restrict__ chains, SweepState st, int s) {
-    const int ci = blockIdx.x;
-    const Chain ch = chains[ci];
-    const LevelGeom g = *ch.geom;
+// Rebuild, after sweep s, of a chain's running bad-block count over the visit order
+// (coarse levels with the bad-block pass): from the first position whose found/bad
+// changed, queue every block whose bad flag no longer matches its count.  Run by one
+// wave; the recurrence (the limit grows with the count) runs over 64 positions at a time.
+__device__ void scan_chain(const Chain* __restrict__ chains, int ci, const SweepState& st, const Queue& qu, int s) {
+    const Chain ch = uniform_load(&chains[ci]);
+    const LevelGeom g = uniform_load(ch.geom);
     if (g.level == 0 || g.bad_sad <= 0) return;
-    const uint32_t start = st.minpos[ci];
+    const uint32_t start = (uint32_t)uniform_int((int)st.minpos[ci]);
     if (start == 0xFFFFFFFFu) return;
     const int n = g.gw * g.gh;
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     int c = 0;
     if (start > 0) {
-        const int p0 = ch.order[start];
-        c = st.cpre[ch.base + (p0 >> 16) * g.gw + (p0 & 0xFFFF)];
+        const int p0 = uniform_int(ch.order[start]);
+        c = uniform_int(st.cpre[ch.base + (p0 >> 16) * g.gw + (p0 & 0xFFFF)]);
     }
     for (int base = (int)start; base < n; base += kWave) {
         const int k = base + lane;
@@ -854,8 +1073,8 @@ __global__ void __launch_bounds__(kWave) sweep_scan_kernel(const Chain* __restri
         int myc = 0, mybad = 0;
         const int cnt = min(kWave, n - base);
         for (int j = 0; j < cnt; j++) {
-            const int ij = __shfl(idx, j);
-            const uint32_t fj = __shfl(f, j);
+            const int ij = rl(idx, j);
+            const uint32_t fj = rlu(f, j);
             const int lim = sat_add_i32(g.bad_sad, sat_mul_i32(g.bad_sad, c) / 16);
             const int bj = (ij > 1 && (int)fj > lim) ? 1 : 0;
             if (lane == j) { myc = c; mybad = bj; }
@@ -863,57 +1082,64 @@ __global__ void __launch_bounds__(kWave) sweep_scan_kernel(const Chain* __restri
         }
         if (k < n) {
             st.cpre[ch.base + idx] = myc;
-            if ((uint32_t)mybad != st.badf[ch.base + idx]) queue_block(st, ci, ch, idx, s);
+            if ((uint32_t)mybad != st.badf[ch.base + idx]) queue_block(st, qu, ci, ch, idx, s);
         }
     }
     if (lane == 0) st.minpos[ci] = 0xFFFFFFFFu;
 }
 
+__global__ void __launch_bounds__(kWave) sweep_scan_kernel(const Chain* __restrict__ chains, SweepState st, int s) {
+    scan_chain(chains, blockIdx.x, st, Queue{nullptr}, s);
+}
+
 // Evaluates block idx of chain ci in sweep s; when its vector (or score) changes, the
 // result is stored and every 3x3 neighbour visited after it is queued for sweep s + 1.
 __device__ __forceinline__ void sweep_item(const Chain* __restrict__ chains, int ci, int idx, const SweepState& st,
-                                           int s) {
+                                           const Queue& qu, int s, lds_u8* lds) {
     const Chain ch = uniform_load(&chains[ci]);
     const LevelGeom g = uniform_load(ch.geom);
     Blk b;
     init_blk(ch, g, b);
     bool bad;
     uint32_t found;
-    const DVec o = search_block(ch, g, b, idx, uniform_int(st.cpre[ch.base + idx]), &bad, &found);
+    const DVec o = search_block(ch, g, b, lds, idx, uniform_int(st.cpre[ch.base + idx]), &bad, &found);
     const DVec old = uniform_load(&ch.res[idx]);
     const bool moved = o.dx != old.dx || o.dy != old.dy || o.score != old.score;
     const uint32_t oldf = (uint32_t)uniform_int((int)st.found[ch.base + idx]);
     const uint32_t oldb = (uint32_t)uniform_int((int)st.badf[ch.base + idx]);
-    if (threadIdx.x == 0) {
+    const int me = uniform_int(ch.opos[idx]);
+    const int lane = lane_id();
+    if (lane == 0) {
         if (moved || o.luma != old.luma || o.done != old.done) ch.res[idx] = o;
         if (found != oldf || (uint32_t)bad != oldb) {
             st.found[ch.base + idx] = found;
             st.badf[ch.base + idx] = bad ? 1u : 0u;
-            if (g.level > 0 && g.bad_sad > 0) atomicMin(&st.minpos[ci], (uint32_t)ch.opos[idx]);
+            if (g.level > 0 && g.bad_sad > 0) atomicMin(&st.minpos[ci], (uint32_t)me);
         }
     }
     if (!moved) return;
-    const int lane = threadIdx.x;
     if (lane < 9 && lane != 4) {
         const int bx = idx % g.gw, by = idx / g.gw;
         const int nx = bx - 1 + lane % 3, ny = by - 1 + lane / 3;
         if (nx >= 0 && ny >= 0 && nx < g.gw && ny < g.gh) {
             const int n = ny * g.gw + nx;
-            if (ch.opos[n] > ch.opos[idx]) queue_block(st, ci, ch, n, s);
+            if (ch.opos[n] > me) queue_block(st, qu, ci, ch, n, s);
         }
     }
 }
 
 // Sweep 0: every block of every chain (grid: blocks x chains).
 __global__ void __launch_bounds__(kWave, 3) sweep0_kernel(const Chain* __restrict__ chains, SweepState st) {
+    __shared__ __attribute__((aligned(16))) uint8_t lds[kLdsPerWave];
     const int ci = blockIdx.y, idx = blockIdx.x;
     const LevelGeom* g = chains[ci].geom;
     if (idx >= g->gw * g->gh) return;
-    sweep_item(chains, ci, idx, st, 0);
+    sweep_item(chains, ci, idx, st, Queue{nullptr}, 0, (lds_u8*)lds);
 }
 
 // Sweep s >= 1 over the blocks queued by sweep s - 1.
 __global__ void __launch_bounds__(kWave, 3) sweep_kernel(const Chain* __restrict__ chains, SweepState st, int s) {
+    __shared__ __attribute__((aligned(16))) uint8_t lds[kLdsPerWave];
     const uint32_t n = (uint32_t)uniform_int((int)__atomic_load_n(&st.counts[s], __ATOMIC_RELAXED));
     const int p = s & 1;
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
@@ -921,8 +1147,69 @@ __global__ void __launch_bounds__(kWave, 3) sweep_kernel(const Chain* __restrict
         // uniform across the wave: keeps the chain's descriptors in scalar registers
         const int ci = __builtin_amdgcn_readfirstlane((int)(item >> 32));
         const int idx = __builtin_amdgcn_readfirstlane((int)(item & 0xFFFFFFFFu));
-        if (threadIdx.x == 0) st.flags[p][chains[ci].base + idx] = 0;
-        sweep_item(chains, ci, idx, st, s);
+        if (lane_id() == 0) st.flags[p][chains[ci].base + idx] = 0;
+        sweep_item(chains, ci, idx, st, Queue{nullptr}, s, (lds_u8*)lds);
+    }
+}
+
+// Hand-over to the per-chain tail before sweep s: its list (global, parity s & 1)
+// moves into per-chain segments wl[q][base + k] of the other parity q = (s + 1) & 1,
+// flagged in flags[q] (clear since sweep s - 1 consumed that list), and
+// tail_count[ci] receives k.  The tail then runs this list as its sweep s + 1.
+__global__ void tail_split_kernel(const Chain* __restrict__ chains, SweepState st, int s) {
+    const uint32_t n = st.counts[s];
+    const int p = s & 1, q2 = (s + 1) & 1;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const unsigned long long item = st.wl[p][i];
+        const int ci = (int)(item >> 32), idx = (int)(item & 0xFFFFFFFFu);
+        const uint32_t base = chains[ci].base;
+        st.flags[p][base + idx] = 0;
+        st.flags[q2][base + idx] = 1;
+        const uint32_t pos = atomicAdd(&st.tail_count[ci], 1u);
+        st.wl[q2][base + pos] = item;
+    }
+}
+
+constexpr int kTailWaves = 8;
+constexpr int kTailCap = 1 << 20;  // sweeps
+
+// The remaining sweeps of one chain in one workgroup: the waves share the chain's
+// list, a workgroup barrier separates the sweeps, and the chain is done when a sweep
+// queues nothing.  Starts from the list that tail_split_kernel left in parity
+// (s0 + 1) & 1, i.e. sweep s0 + 1.
+__global__ void __launch_bounds__(kWave * kTailWaves) tail_kernel(const Chain* __restrict__ chains, SweepState st,
+                                                                  int s0) {
+    __shared__ uint32_t cnt[2];
+    __shared__ __attribute__((aligned(16))) uint8_t lds[kTailWaves][kLdsPerWave];
+    const int ci = blockIdx.x;
+    const int wave = threadIdx.x / kWave;
+    const uint32_t base = chains[ci].base;
+    int s = s0 + 1;
+    if (threadIdx.x == 0) {
+        cnt[s & 1] = st.tail_count[ci];
+        cnt[(s + 1) & 1] = 0;
+    }
+    __syncthreads();
+    for (;; s++) {
+        const int p = s & 1;
+        const uint32_t n = cnt[p];
+        if (n == 0) break;  // uniform: read after the barrier
+        if (s - s0 > kTailCap) {  // never reached (each sweep settles the first dirty position); reported to the host
+            if (threadIdx.x == 0) st.tail_count[ci] = 0xFFFFFFFFu;
+            break;
+        }
+        const Queue qu{&cnt[(s + 1) & 1]};
+        for (uint32_t i = wave; i < n; i += kTailWaves) {
+            const unsigned long long item = st.wl[p][base + i];
+            const int idx = __builtin_amdgcn_readfirstlane((int)(item & 0xFFFFFFFFu));
+            if (lane_id() == 0) st.flags[p][base + idx] = 0;
+            sweep_item(chains, ci, idx, st, qu, s, (lds_u8*)lds[wave]);
+        }
+        __syncthreads();
+        if (wave == 0) scan_chain(chains, ci, st, qu, s);
+        __syncthreads();
+        if (threadIdx.x == 0) cnt[p] = 0;
+        __syncthreads();
     }
 }
 
@@ -1238,6 +1525,12 @@ hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const Sw
 
 hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s) {
     hipLaunchKernelGGL(sweep_kernel, dim3(grid), dim3(kWave), 0, s, d_chains, st, sweep);
+    return hipGetLastError();
+}
+
+hipError_t launch_tail(const Chain* d_chains, int nchains, const SweepState& st, int sweep, hipStream_t s) {
+    hipLaunchKernelGGL(tail_split_kernel, dim3(64), dim3(256), 0, s, d_chains, st, sweep);
+    hipLaunchKernelGGL(tail_kernel, dim3(nchains), dim3(kWave * kTailWaves), 0, s, d_chains, st, sweep);
     return hipGetLastError();
 }
 

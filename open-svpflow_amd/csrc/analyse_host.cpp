@@ -15,6 +15,7 @@ hipError_t launch_sweep_init(const Chain* d_chains, int nchains, int max_n, cons
 hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s);
 hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s);
 hipError_t launch_sweep_scan(const Chain* d_chains, int nchains, const SweepState& st, int sweep, hipStream_t s);
+hipError_t launch_tail(const Chain* d_chains, int nchains, const SweepState& st, int sweep, hipStream_t s);
 hipError_t launch_interp(const void* d_tr, int ntr, int max_n, hipStream_t s);
 hipError_t launch_global(const void* d_jobs, int njobs, hipStream_t s);
 hipError_t launch_sort_keys(const void* d_jobs, int njobs, int max_n, hipStream_t s);
@@ -145,6 +146,7 @@ struct ClipPlan {
 };
 
 constexpr int kMaxSweeps = 4096;
+constexpr int kTailPerChain = 4;  // hand over to the per-chain tail below this many queued blocks per chain
 constexpr int kNotConverged = 1;
 
 // Parallel sweeps of one level until no block changes (each sweep evaluates the blocks
@@ -153,8 +155,15 @@ constexpr int kNotConverged = 1;
 // sequential result.  The host checks the queue length after each chunk of sweeps;
 // sweeps launched past convergence find an empty queue and return at once.
 int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total_blocks, const SweepState& st,
-               bool scan) {
-    if (!ctx->h_count) SVPF_HIP(hipHostMalloc((void**)&ctx->h_count, 64, hipHostMallocDefault));
+               bool scan, int level) {
+    const size_t words = (size_t)std::max(kMaxSweeps + 2, nch);
+    if (ctx->h_count_cap < words) {
+        if (ctx->h_count) (void)hipHostFree(ctx->h_count);
+        ctx->h_count = nullptr;
+        ctx->h_count_cap = 0;
+        SVPF_HIP(hipHostMalloc((void**)&ctx->h_count, 4 * words, hipHostMallocDefault));
+        ctx->h_count_cap = words;
+    }
     SVPF_HIP(launch_sweep0(dch, nch, max_n, st, ctx->stream));
     if (scan) SVPF_HIP(launch_sweep_scan(dch, nch, st, 0, ctx->stream));
     const int grid = (int)std::min<size_t>(std::max<size_t>(total_blocks, 1), 4096);
@@ -168,8 +177,38 @@ int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total
         s += chunk;
         SVPF_HIP(hipMemcpyAsync(ctx->h_count, st.counts + s, 4, hipMemcpyDeviceToHost, ctx->stream));
         SVPF_HIP(hipStreamSynchronize(ctx->stream));
-        ctx->sweeps_last += chunk;
-        if (*ctx->h_count == 0) return SVPF_OK;
+        if (*ctx->h_count != 0 && *ctx->h_count <= (uint32_t)nch * kTailPerChain) {
+            // short lists: each chain finishes in its own workgroup, no launches between sweeps
+            SVPF_HIP(launch_tail(dch, nch, st, s, ctx->stream));
+            SVPF_HIP(hipMemcpyAsync(ctx->h_count, st.tail_count, 4 * (size_t)nch, hipMemcpyDeviceToHost, ctx->stream));
+            SVPF_HIP(hipStreamSynchronize(ctx->stream));
+            for (int c = 0; c < nch; c++)
+                if (ctx->h_count[c] == 0xFFFFFFFFu) return kNotConverged;
+            if (ctx->prof) {
+                long long* t = ctx->sweep_stats[std::min(level, 15)];
+                t[0] += 1;
+                t[1] += s;
+                t[2] += s;
+            }
+            return SVPF_OK;
+        }
+        if (*ctx->h_count == 0) {
+            if (ctx->prof) {  // statistics for the benchmark
+                SVPF_HIP(hipMemcpy(ctx->h_count, st.counts, 4 * (size_t)(s + 1), hipMemcpyDeviceToHost));
+                long long* t = ctx->sweep_stats[std::min(level, 15)];
+                t[0] += 1;
+                t[1] += s;
+                long long ev = (long long)total_blocks;
+                int last = 0;
+                for (int k = 1; k <= s; k++) {
+                    ev += ctx->h_count[k];
+                    if (ctx->h_count[k]) last = k;
+                }
+                t[2] += last + 1;
+                t[3] += ev;
+            }
+            return SVPF_OK;
+        }
         chunk = std::min(chunk * 2, 64);
     }
 }
@@ -299,7 +338,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
     }
     // sweep state per level: flags x2 (one entry per block of every chain), worklists x2, counters
     size_t level_blocks[16] = {}, flags_off[16][2], wl_off[16][2], counts_off[16], found_off[16], badf_off[16],
-        cpre_off[16], minpos_off[16];
+        cpre_off[16], minpos_off[16], tailc_off[16];
     int level_max_n[16] = {};
     for (int l = 0; l < max_levels; l++) {
         size_t tot = 0;
@@ -321,6 +360,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
         badf_off[l] = work.take(4 * tot + 4);
         cpre_off[l] = work.take(4 * tot + 4);
         minpos_off[l] = work.take(4 * (size_t)nchains + 4);
+        tailc_off[l] = work.take(4 * (size_t)nchains + 4);
     }
     for (int l = 0; l < max_levels; l++) {
         size_t tot = 0;
@@ -580,10 +620,11 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
             st.badf = (uint32_t*)(W + badf_off[l]);
             st.cpre = (int32_t*)(W + cpre_off[l]);
             st.minpos = (uint32_t*)(W + minpos_off[l]);
+            st.tail_count = (uint32_t*)(W + tailc_off[l]);
             st.max_sweeps = kMaxSweeps;
             SVPF_HIP(launch_sweep_init(dch, nch_step[l], level_max_n[l], st, ctx->stream));
             int rc2 = seq_step[l] ? SVPF_OK
-                                  : run_sweeps(ctx, dch, nch_step[l], level_max_n[l], level_blocks[l], st, bad_step[l]);
+                                  : run_sweeps(ctx, dch, nch_step[l], level_max_n[l], level_blocks[l], st, bad_step[l], l);
             if (rc2 == kNotConverged || seq_step[l])
                 SVPF_HIP(launch_search_level(dch, nch_step[l], ctx->stream));
             else if (rc2)

@@ -321,6 +321,13 @@ int svpf_profile_enable(svpf_ctx* ctx, int on) {
     SVPF_HIP(hipStreamSynchronize(ctx->stream));
     prof_clear(ctx);
     ctx->prof = on != 0;
+    std::memset(ctx->sweep_stats, 0, sizeof ctx->sweep_stats);
+    return SVPF_OK;
+}
+
+int svpf_analyse_stats(svpf_ctx* ctx, int32_t level, int64_t* out4) {
+    if (!ctx || !out4 || level < 0 || level >= 16) return fail(SVPF_ERR_ARG, "svpf_analyse_stats: bad argument");
+    for (int k = 0; k < 4; k++) out4[k] = ctx->sweep_stats[level][k];
     return SVPF_OK;
 }
 

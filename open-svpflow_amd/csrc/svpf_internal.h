@@ -24,8 +24,11 @@ struct svpf_ctx {
     size_t work_cap = 0;
     void* d_sort_tmp = nullptr;
     size_t sort_tmp_cap = 0;
-    uint32_t* h_count = nullptr;  // pinned word for reading back sweep queue lengths
-    long long sweeps_last = 0;    // sweeps launched by Analyse calls (statistics)
+    uint32_t* h_count = nullptr;  // pinned words for reading back sweep queue lengths
+    size_t h_count_cap = 0;
+    // Analyse sweep statistics per level since profiling was last enabled:
+    // {level runs, sweeps launched, sweeps with work, block evaluations}
+    long long sweep_stats[16][4] = {};
     // Optional kernel timing: HIP event pairs recorded around launches on `stream`.
     struct ProfMark {
         int kernel;

@@ -68,6 +68,7 @@ def lib() -> C.CDLL:
                                              C.POINTER(Frame)]
         L.svpf_profile_enable.argtypes = [C.c_void_p, C.c_int]
         L.svpf_profile_read.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+        L.svpf_analyse_stats.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64)]
         for bind in _BINDERS:
             bind(L)
         _lib = L
@@ -156,6 +157,12 @@ class Context:
 
     def profile(self, on: bool) -> None:
         check(lib().svpf_profile_enable(self.h, 1 if on else 0))
+
+    def analyse_stats(self, level: int):
+        """{runs, sweeps launched, sweeps with work, block evaluations} of one level."""
+        out = (C.c_int64 * 4)()
+        check(lib().svpf_analyse_stats(self.h, level, out))
+        return dict(zip(("runs", "sweeps", "busy_sweeps", "evaluations"), list(out)))
 
     def profile_read(self, kernel: str):
         """(total ms, launches) of one timed kernel since profiling was enabled."""
