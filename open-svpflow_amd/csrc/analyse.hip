@@ -124,120 +124,12 @@ __device__ __forceinline__ uint32_t xor_sum8(uint32_t v) {
     return v;
 }
 
-// ---- LDS copy of one block's source and reference search window (one region per wave)
-
-typedef __attribute__((address_space(3))) uint8_t lds_u8;
-typedef __attribute__((address_space(3))) uint32_t __attribute__((aligned(1))) lds_u32u;
-typedef __attribute__((address_space(3))) uint32_t lds_u32;
-
-constexpr int kWinBytes = 12288;  // reference window budget per wave
-constexpr int kSrcBytes = 1536;   // source block: luma + 2 chroma, up to 32x32
-constexpr int kLdsPerWave = kWinBytes + kSrcBytes;
-
-__device__ __forceinline__ uint32_t lds_ld32(const lds_u8* p) { return *reinterpret_cast<const lds_u32u*>(p); }
-
-// The staged window (wave-uniform).  Candidates whose interior luma and chroma blocks
-// lie inside it read LDS; any other candidate reads the planes as before.
-struct Win {
-    int on;
-    int lx0, ly0, lw, lh, lpitch;  // luma: level pixels [lx0, lx0 + lw) x [ly0, ly0 + lh) per slot
-    int cx0, cy0, cw, ch, cpitch;  // chroma
-    int ref_u, ref_v;              // LDS offsets of the chroma windows (luma window at 0)
-    int src_l, src_u, src_v;       // LDS offsets of the source blocks (pitch bw / cw)
-};
-
-// Copies rows [y0, y0 + h) x columns [x0, x0 + w) of one plane region (level pixels,
-// first plane row `row0`, level size lw x lh) into LDS at `dst` with `pitch`; pixels
-// outside the level are written as 0 (no candidate that reads LDS uses them).
-__device__ __forceinline__ void stage_rows(lds_u8* dst, int pitch, const uint8_t* plane, int plen, int stride, int row0,
-                                           int x0, int y0, int w, int h, int lw, int lh) {
-    const int wd = pitch >> 2;
-    for (int e = lane_id(); e < h * wd; e += kWave) {
-        const int r = e / wd, c = (e - r * wd) * 4;
-        const int gy = y0 + r, gx = x0 + c;
-        uint32_t v = 0;
-        if (gy >= 0 && gy < lh) {
-            const int base = (row0 + gy) * stride;
-            if (gx >= 0 && gx + 4 <= lw) {
-                v = ld32(plane + base + gx);
-            } else {
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const int x = gx + k;
-                    if (x >= 0 && x < lw) v |= (uint32_t)rdb(plane, plen, base + x, 0) << (8 * k);
-                }
-            }
-        }
-        *reinterpret_cast<lds_u32*>(dst + r * pitch + c) = v;
-    }
-}
-
-// Stages the block's source and the reference window covering the vectors within
-// +-V of (cx, cy) (vector units), shrinking V to the LDS budget.  Wave-cooperative.
-__device__ __forceinline__ void stage_window(const Blk& b, lds_u8* lds, int cx, int cy, int V, bool chroma, Win& w) {
-    w.on = 0;
-    if (!lds || !pow2_4_32(b.bw) || b.bw != b.bh) return;
-    const bool subpel = b.level == 0 && b.pel >= 2;
-    const int sh = subpel ? 1 : 0, csh = subpel ? 2 : 1;
-    const int nsub = subpel ? 4 : 1;
-    const int ipx = b.px - ((b.level > 0 && b.px + b.bw >= b.lw) ? 1 : 0);
-    const int ipy = b.py - ((b.level > 0 && b.py + b.bh >= b.lh) ? 1 : 0);
-    const int cwb = b.bw / 2, chb = b.bh / 2;
-    const bool cuse = chroma && cwb >= 4 && chb >= 4;
-    const int cpx = b.px / 2, cpy = b.py / 2;
-    for (; V >= 2; V >>= 1) {
-        w.lx0 = ipx + ((cx - V) >> sh);
-        w.ly0 = ipy + ((cy - V) >> sh);
-        w.lw = b.bw + ((cx + V) >> sh) - ((cx - V) >> sh);
-        w.lh = b.bh + ((cy + V) >> sh) - ((cy - V) >> sh);
-        w.lpitch = (w.lw + 3) & ~3;
-        w.cx0 = cpx + ((cx - V) >> csh);
-        w.cy0 = cpy + ((cy - V) >> csh);
-        w.cw = cwb + ((cx + V) >> csh) - ((cx - V) >> csh);
-        w.ch = chb + ((cy + V) >> csh) - ((cy - V) >> csh);
-        w.cpitch = (w.cw + 3) & ~3;
-        const int lbytes = nsub * w.lh * w.lpitch;
-        const int cbytes = cuse ? nsub * w.ch * w.cpitch : 0;
-        if (lbytes + 2 * cbytes <= kWinBytes) {
-            w.ref_u = lbytes;
-            w.ref_v = lbytes + cbytes;
-            w.on = 1;
-            break;
-        }
-    }
-    if (!w.on) return;
-    w.src_l = kWinBytes;
-    w.src_u = kWinBytes + b.bw * b.bh;
-    w.src_v = w.src_u + cwb * chb;
-    for (int k = 0; k < nsub; k++) {
-        const int sub = (k >> 1) * b.pel + (k & 1);
-        stage_rows(lds + k * w.lh * w.lpitch, w.lpitch, b.rp[0], b.rl[0], b.rs[0], b.y_off + sub * b.lh, w.lx0, w.ly0,
-                   w.lw, w.lh, b.lw, b.lh);
-        if (cuse) {
-            const int crow = b.c_off + (subpel ? sub * b.clh : 0);
-            stage_rows(lds + w.ref_u + k * w.ch * w.cpitch, w.cpitch, b.rp[1], b.rl[1], b.rs[1], crow, w.cx0, w.cy0, w.cw,
-                       w.ch, b.clw, b.clh);
-            stage_rows(lds + w.ref_v + k * w.ch * w.cpitch, w.cpitch, b.rp[2], b.rl[2], b.rs[2], crow, w.cx0, w.cy0, w.cw,
-                       w.ch, b.clw, b.clh);
-        }
-    }
-    stage_rows(lds + w.src_l, b.bw, b.sp[0], b.sl[0], b.ss[0], b.y_off, ipx, ipy, b.bw, b.bh, b.lw, b.lh);
-    if (cuse) {
-        stage_rows(lds + w.src_u, cwb, b.sp[1], b.sl[1], b.ss[1], b.c_off, cpx, cpy, cwb, chb, b.clw, b.clh);
-        stage_rows(lds + w.src_v, cwb, b.sp[2], b.sl[2], b.ss[2], b.c_off, cpx, cpy, cwb, chb, b.clw, b.clh);
-    }
-    // the lanes read what other lanes of the wave wrote
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // One candidate's cost computed by the 8 lanes of a group (part = lane & 7); every
 // lane of the group receives (L, C).  Blocks whose luma and chroma lie inside their
 // levels (the common case) split into units -- 4x4 tiles for SATD, rows for SAD --
 // spread over the group; any other path is computed whole by part 0 (cand_cost_full).
-__device__ __forceinline__ void group_cost(const Blk& b, const lds_u8* lds, const Win& win, int mvx, int mvy,
-                                           bool chroma, int part, uint32_t& Lout, uint32_t& Cout) {
+__device__ __forceinline__ void group_cost(const Blk& b, int mvx, int mvy, bool chroma, int part, uint32_t& Lout,
+                                           uint32_t& Cout) {
     const int pel = b.pel;
     const int b0 = -pel * b.px, b1 = -pel * b.py;
     const int b2 = pel * (b.lw - b.px - b.bw) + 1, b3 = pel * (b.lh - b.py - b.bh) + 1;
@@ -261,58 +153,8 @@ __device__ __forceinline__ void group_cost(const Blk& b, const lds_u8* lds, cons
     const bool cfast = cw == ch && (cw == 4 || cw == 8 || cw == 16) && cpx + cw <= b.clw && cpy + ch <= b.clh &&
                        crx >= 0 && cry >= 0 && crx + cw <= b.clw && cry + ch <= b.clh;
     const bool fast = lfast && (!cuse || cfast);
-    const bool inwin = fast && win.on && frx >= win.lx0 && frx + b.bw <= win.lx0 + win.lw && fry >= win.ly0 &&
-                       fry + b.bh <= win.ly0 + win.lh &&
-                       (!cuse || (crx >= win.cx0 && crx + cw <= win.cx0 + win.cw && cry >= win.cy0 &&
-                                  cry + ch <= win.cy0 + win.ch));
     uint32_t a0 = 0, a1 = 0, a2 = 0;  // luma, u, v partial sums (fast) or (L, C, 0) from part 0
-    if (inwin) {
-        // the same units as below, read from the staged copies
-        const int slot = subpel ? ((mvy & 1) * 2 + (mvx & 1)) : 0;
-        const lds_u8* ls = lds + win.src_l;
-        const lds_u8* lr = lds + (slot * win.lh + (fry - win.ly0)) * win.lpitch + (frx - win.lx0);
-        const lds_u8* us = lds + win.src_u;
-        const lds_u8* ur = lds + win.ref_u + (slot * win.ch + (cry - win.cy0)) * win.cpitch + (crx - win.cx0);
-        const lds_u8* vs = lds + win.src_v;
-        const lds_u8* vr = lds + win.ref_v + (slot * win.ch + (cry - win.cy0)) * win.cpitch + (crx - win.cx0);
-        if (sq) {
-            const int lt = b.bw >> 2, ct = cw >> 2;
-            const int nl = lt * lt, nc = cuse ? ct * ct : 0;
-            for (int u = part; u < nl + 2 * nc; u += kGroup) {
-                const int plane = u < nl ? 0 : (u < nl + nc ? 1 : 2);
-                const int t = plane == 0 ? u : (plane == 1 ? u - nl : u - nl - nc);
-                const int per = plane == 0 ? lt : ct;
-                const int ty = t / per, tx = t - ty * per;
-                const lds_u8* sp = plane == 0 ? ls : (plane == 1 ? us : vs);
-                const lds_u8* rp = plane == 0 ? lr : (plane == 1 ? ur : vr);
-                const int ss = plane == 0 ? b.bw : cw;
-                const int rs = plane == 0 ? win.lpitch : win.cpitch;
-                uint32_t x[4], y[4];
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    x[k] = lds_ld32(sp + (ty * 4 + k) * ss + tx * 4);
-                    y[k] = lds_ld32(rp + (ty * 4 + k) * rs + tx * 4);
-                }
-                const uint32_t h = had4_rows_pk(x, y);
-                if (plane == 0) a0 += h; else if (plane == 1) a1 += h; else a2 += h;
-            }
-        } else {
-            const int nl = b.bh, nc = cuse ? ch : 0;
-            for (int u = part; u < nl + 2 * nc; u += kGroup) {
-                const int plane = u < nl ? 0 : (u < nl + nc ? 1 : 2);
-                const int r = plane == 0 ? u : (plane == 1 ? u - nl : u - nl - nc);
-                const lds_u8* sp = plane == 0 ? ls : (plane == 1 ? us : vs);
-                const lds_u8* rp = plane == 0 ? lr : (plane == 1 ? ur : vr);
-                const int ss = plane == 0 ? b.bw : cw;
-                const int rs = plane == 0 ? win.lpitch : win.cpitch;
-                const int words = (plane == 0 ? b.bw : cw) >> 2;
-                uint32_t acc = 0;
-                for (int w = 0; w < words; w++)
-                    acc = __builtin_amdgcn_sad_u8(lds_ld32(sp + r * ss + 4 * w), lds_ld32(rp + r * rs + 4 * w), acc);
-                if (plane == 0) a0 += acc; else if (plane == 1) a1 += acc; else a2 += acc;
-            }
-        }
-    } else if (fast) {
+    if (fast) {
         const int rrow = b.y_off + sub * b.lh;
         const int cref = b.c_off + ((b.level == 0 && pel > 1) ? sub * b.clh : 0);
         const uint8_t* ls = b.sp[0] + (int64_t)(b.y_off + ipy) * b.ss[0] + ipx;
@@ -601,8 +443,8 @@ enum Step : int {
 // predictor -- exactly what the reference's `processed` grid selects in its visit
 // order.  Returns the block's vector (uniform across the wave), *bad = whether the
 // coarse-level bad-block expansion ran and *found_out = the cost before it.
-__device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g, Blk& b, lds_u8* lds, int idx,
-                                             int bad_count, bool* bad, uint32_t* found_out) {
+__device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g, Blk& b, int idx,
+                                             int bad_count, bool* bad, uint32_t* found_out, Memo* memo) {
     const int lane = lane_id();
     const int part = lane & (kGroup - 1);
     const int gx = ch.global ? uniform_int(ch.global[0]) : 0, gy = ch.global ? uniform_int(ch.global[1]) : 0;
@@ -695,8 +537,6 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
 
     Best best{0, 0, 0xFFFFFFFFu, INT64_MAX};
     Search cur = s;
-    Win wnd;  // staged search window
-    wnd.on = 0;
     Pattern P;
     bool pchroma;
     int step;
@@ -715,6 +555,10 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
     int bmx = 0, bmy = 0, dir = -2, hi = 0, odir = 0, hex_r = 0, exp_next = kStRefineDone, ring = 0;
     uint32_t found = 0;
     bool is_bad = false;
+    // memo of the refinement (blocks off the edge row and column only)
+    if (edge) memo = nullptr;
+    bool memo_hit = false;
+    int key_x = 0, key_y = 0;
     for (;;) {
         // ---------------- the evaluation site: kPerPass candidates per pass, one group each
         int win = -1;
@@ -734,7 +578,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                     valid = in_bounds(cur, mx, my);
                 }
                 uint32_t L = 0, C = 0;
-                if (valid) group_cost(b, lds, wnd, mx, my, pchroma, part, L, C);
+                if (valid) group_cost(b, mx, my, pchroma, part, L, C);
                 const uint32_t sad = sat_add(L, C);
                 int64_t cost = INT64_MAX, motion = INT64_MAX;
                 if (valid) {
@@ -819,13 +663,19 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                         if (pc < (field_valid ? fc : INT64_MAX)) { cur.pmx = prx; cur.pmy = pry; }
                     }
                     best.cost = (int64_t)best.sad;
+                    key_x = best.mvx;
+                    key_y = best.mvy;
+                    if (memo) {
+                        const Memo m = uniform_load(memo);
+                        if (m.valid && m.kx == key_x && m.ky == key_y) {
+                            best = Best{m.rx, m.ry, m.rsad, m.rcost};
+                            memo_hit = true;
+                            step = kStRefineDone;
+                            break;
+                        }
+                    }
                     const int stype = (edge && level == 1) ? 0 : g.search_type;
                     exp_next = kStRefineDone;
-                    if (stype >= 2 && stype <= 4) {
-                        // the refine searches stay near the best predictor: stage its window
-                        const int reach = stype == 4 ? radius : (stype == 3 ? 2 * radius + 4 : radius + 4);
-                        stage_window(b, lds, best.mvx, best.mvy, reach, !smallest, wnd);
-                    }
                     if (stype == 2) {
                         hex_r = radius;
                         step = kStHexBegin;
@@ -910,6 +760,9 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                     step = kStRefineDone;
                     break;
                 case kStRefineDone: {
+                    if (memo && !memo_hit && lane == 0)
+                        *memo = Memo{(int16_t)key_x, (int16_t)key_y, (int16_t)best.mvx, (int16_t)best.mvy, best.sad, 1u,
+                                     best.cost};
                     // bad-block expansion at coarse levels (:607-672), around the zero vector
                     found = best.sad;
                     const int bad_limit = sat_add_i32(g.bad_sad, sat_mul_i32(g.bad_sad, bad_count) / 16);
@@ -978,7 +831,6 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
 // chain.  Used where the parallel sweeps do not apply (the coarse-level bad-block
 // pass keeps a running count over the visit order) and as their fallback.
 __global__ void __launch_bounds__(kWave, 3) search_level_kernel(const Chain* __restrict__ chains) {
-    __shared__ __attribute__((aligned(16))) uint8_t lds[kLdsPerWave];
     const Chain ch = uniform_load(&chains[blockIdx.x]);
     const LevelGeom g = uniform_load(ch.geom);
     Blk b;
@@ -992,7 +844,7 @@ __global__ void __launch_bounds__(kWave, 3) search_level_kernel(const Chain* __r
         const int idx = by * g.gw + bx;
         bool bad;
         uint32_t found;
-        const DVec o = search_block(ch, g, b, (lds_u8*)lds, idx, bad_count, &bad, &found);
+        const DVec o = search_block(ch, g, b, idx, bad_count, &bad, &found, nullptr);
         bad_count += bad ? 1 : 0;
         if (threadIdx.x == 0) ch.res[idx] = o;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -1015,6 +867,7 @@ __global__ void sweep_init_kernel(const Chain* __restrict__ chains, SweepState s
         st.found[ch.base + i] = 0;
         st.badf[ch.base + i] = 0;
         st.cpre[ch.base + i] = 1 << 30;  // sweep 0 runs no bad-block pass: the limit saturates
+        st.memo[ch.base + i].valid = 0;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st.minpos[blockIdx.y] = 0;  // the first rebuild covers the whole order
@@ -1095,14 +948,15 @@ __global__ void __launch_bounds__(kWave) sweep_scan_kernel(const Chain* __restri
 // Evaluates block idx of chain ci in sweep s; when its vector (or score) changes, the
 // result is stored and every 3x3 neighbour visited after it is queued for sweep s + 1.
 __device__ __forceinline__ void sweep_item(const Chain* __restrict__ chains, int ci, int idx, const SweepState& st,
-                                           const Queue& qu, int s, lds_u8* lds) {
+                                           const Queue& qu, int s) {
     const Chain ch = uniform_load(&chains[ci]);
     const LevelGeom g = uniform_load(ch.geom);
     Blk b;
     init_blk(ch, g, b);
     bool bad;
     uint32_t found;
-    const DVec o = search_block(ch, g, b, lds, idx, uniform_int(st.cpre[ch.base + idx]), &bad, &found);
+    const DVec o = search_block(ch, g, b, idx, uniform_int(st.cpre[ch.base + idx]), &bad, &found,
+                                st.memo + ch.base + idx);
     const DVec old = uniform_load(&ch.res[idx]);
     const bool moved = o.dx != old.dx || o.dy != old.dy || o.score != old.score;
     const uint32_t oldf = (uint32_t)uniform_int((int)st.found[ch.base + idx]);
@@ -1130,16 +984,14 @@ __device__ __forceinline__ void sweep_item(const Chain* __restrict__ chains, int
 
 // Sweep 0: every block of every chain (grid: blocks x chains).
 __global__ void __launch_bounds__(kWave, 3) sweep0_kernel(const Chain* __restrict__ chains, SweepState st) {
-    __shared__ __attribute__((aligned(16))) uint8_t lds[kLdsPerWave];
     const int ci = blockIdx.y, idx = blockIdx.x;
     const LevelGeom* g = chains[ci].geom;
     if (idx >= g->gw * g->gh) return;
-    sweep_item(chains, ci, idx, st, Queue{nullptr}, 0, (lds_u8*)lds);
+    sweep_item(chains, ci, idx, st, Queue{nullptr}, 0);
 }
 
 // Sweep s >= 1 over the blocks queued by sweep s - 1.
 __global__ void __launch_bounds__(kWave, 3) sweep_kernel(const Chain* __restrict__ chains, SweepState st, int s) {
-    __shared__ __attribute__((aligned(16))) uint8_t lds[kLdsPerWave];
     const uint32_t n = (uint32_t)uniform_int((int)__atomic_load_n(&st.counts[s], __ATOMIC_RELAXED));
     const int p = s & 1;
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
@@ -1148,7 +1000,7 @@ __global__ void __launch_bounds__(kWave, 3) sweep_kernel(const Chain* __restrict
         const int ci = __builtin_amdgcn_readfirstlane((int)(item >> 32));
         const int idx = __builtin_amdgcn_readfirstlane((int)(item & 0xFFFFFFFFu));
         if (lane_id() == 0) st.flags[p][chains[ci].base + idx] = 0;
-        sweep_item(chains, ci, idx, st, Queue{nullptr}, s, (lds_u8*)lds);
+        sweep_item(chains, ci, idx, st, Queue{nullptr}, s);
     }
 }
 
@@ -1180,7 +1032,6 @@ constexpr int kTailCap = 1 << 20;  // sweeps
 __global__ void __launch_bounds__(kWave * kTailWaves) tail_kernel(const Chain* __restrict__ chains, SweepState st,
                                                                   int s0) {
     __shared__ uint32_t cnt[2];
-    __shared__ __attribute__((aligned(16))) uint8_t lds[kTailWaves][kLdsPerWave];
     const int ci = blockIdx.x;
     const int wave = threadIdx.x / kWave;
     const uint32_t base = chains[ci].base;
@@ -1203,7 +1054,7 @@ __global__ void __launch_bounds__(kWave * kTailWaves) tail_kernel(const Chain* _
             const unsigned long long item = st.wl[p][base + i];
             const int idx = __builtin_amdgcn_readfirstlane((int)(item & 0xFFFFFFFFu));
             if (lane_id() == 0) st.flags[p][base + idx] = 0;
-            sweep_item(chains, ci, idx, st, qu, s, (lds_u8*)lds[wave]);
+            sweep_item(chains, ci, idx, st, qu, s);
         }
         __syncthreads();
         if (wave == 0) scan_chain(chains, ci, st, qu, s);

@@ -56,6 +56,19 @@ struct Chain {
     uint32_t base;         // first index of this chain's blocks in the level's flag arrays
 };
 
+// Result of a block's refinement search, keyed by the predictor winner it started
+// from.  For a block off the level's edge row and column, the refinement depends only
+// on that start vector (its predictor, lambda, radius and bounds come from the level's
+// fixed inputs), so a re-evaluation whose predictor list picks the same winner reuses
+// the stored result instead of searching again.
+struct alignas(8) Memo {
+    int16_t kx, ky;   // predictor winner (start of the refinement)
+    int16_t rx, ry;   // refined vector
+    uint32_t rsad;    // its sad
+    uint32_t valid;
+    int64_t rcost;    // its cost (the bad-block expansion compares against it)
+};
+
 // Worklists of the parallel sweeps of one level (all chains together).  Sweep s
 // processes wl[s & 1][0 .. counts[s]) whose members are flagged in flags[s & 1]; a
 // block whose result changes queues its later-visited 3x3 neighbours for sweep s + 1.
@@ -73,6 +86,7 @@ struct SweepState {
     int32_t* cpre;              // per block
     uint32_t* minpos;           // per chain
     uint32_t* tail_count;       // per chain: list length handed to the per-chain tail
+    Memo* memo;                 // per block
     int max_sweeps;
 };
 
