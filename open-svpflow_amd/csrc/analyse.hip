@@ -897,52 +897,99 @@ __device__ __forceinline__ void queue_block(const SweepState& st, const Queue& q
     }
 }
 
+constexpr int kScanThreads = 512;  // threads of a workgroup that rebuilds one chain's count
+
+// Exclusive prefix sum of v over the workgroup's threads (wsum: one int per wave).
+__device__ __forceinline__ int block_excl_scan(int v, int* wsum) {
+    const int lane = lane_id(), w = (int)(threadIdx.x / kWave);
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == kWave - 1) wsum[w] = x;
+    __syncthreads();
+    int base = 0;
+    for (int j = 0; j < w; j++) base += wsum[j];
+    __syncthreads();
+    return base + x - v;
+}
+
+// The running bad-block count after positions [lo, hi) of the visit order, from input
+// c.  fseq holds each position's pre-expansion cost, 0 for blocks that never count
+// (the first two raster blocks): with bad_sad > 0 the limit is at least bad_sad.
+__device__ __forceinline__ int scan_walk(const uint32_t* fseq, int lo, int hi, int bad_sad, int c) {
+    for (int k = lo; k < hi; k++)
+        if ((int)fseq[k] > sat_add_i32(bad_sad, sat_mul_i32(bad_sad, c) / 16)) c++;
+    return c;
+}
+
 // Rebuild, after sweep s, of a chain's running bad-block count over the visit order
-// (coarse levels with the bad-block pass): from the first position whose found/bad
-// changed, queue every block whose bad flag no longer matches its count.  Run by one
-// wave; the recurrence (the limit grows with the count) runs over 64 positions at a time.
-__device__ void scan_chain(const Chain* __restrict__ chains, int ci, const SweepState& st, const Queue& qu, int s) {
+// (coarse levels with the bad-block pass, :607-611): from the first position whose
+// found/bad changed, cpre is recomputed and every block whose bad flag no longer
+// matches its count is queued.  Run by a whole workgroup of kScanThreads threads
+// (sh: kScanThreads + kScanThreads / kWave ints of LDS).
+//
+// The recurrence c' = c + [f > limit(c)] is split into one chunk of positions per
+// thread.  A chunk's input count is found by iterating
+//     in[t] = in[0] + sum over u < t of (walk(u, in[u]) - in[u])
+// from the previous rebuild's counts: chunk 0's input is exact, so chunk t is exact
+// after t iterations and a repeat with no change is the sequential result (usually
+// after one or two iterations, since a sweep changes few costs).
+__device__ void scan_chain(const Chain* __restrict__ chains, int ci, const SweepState& st, const Queue& qu, int s,
+                           int* sh) {
     const Chain ch = uniform_load(&chains[ci]);
     const LevelGeom g = uniform_load(ch.geom);
     if (g.level == 0 || g.bad_sad <= 0) return;
-    const uint32_t start = (uint32_t)uniform_int((int)st.minpos[ci]);
+    const uint32_t start = __hip_atomic_load(&st.minpos[ci], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();  // every thread has read minpos before it is cleared below
     if (start == 0xFFFFFFFFu) return;
+    const int T = (int)blockDim.x, tid = (int)threadIdx.x;
     const int n = g.gw * g.gh;
-    const int lane = lane_id();
-    int c = 0;
+    const int m = (n - (int)start + T - 1) / T;
+    const int lo = min((int)start + tid * m, n), hi = min(lo + m, n);
+    uint32_t* fs = st.fseq + ch.base;
+    int c0 = 0;
     if (start > 0) {
-        const int p0 = uniform_int(ch.order[start]);
-        c = uniform_int(st.cpre[ch.base + (p0 >> 16) * g.gw + (p0 & 0xFFFF)]);
+        const int p0 = ch.order[start];
+        c0 = st.cpre[ch.base + (p0 >> 16) * g.gw + (p0 & 0xFFFF)];
     }
-    for (int base = (int)start; base < n; base += kWave) {
-        const int k = base + lane;
-        int idx = -1;
-        uint32_t f = 0;
-        if (k < n) {
-            const int pk = ch.order[k];
-            idx = (pk >> 16) * g.gw + (pk & 0xFFFF);
-            f = st.found[ch.base + idx];
+    int guess = c0;
+    for (int k = lo; k < hi; k++) {
+        const int pk = ch.order[k];
+        const int idx = (pk >> 16) * g.gw + (pk & 0xFFFF);
+        if (k == lo && tid > 0) {
+            const int old = st.cpre[ch.base + idx];
+            guess = old < (1 << 30) ? old : c0;  // before the first rebuild of the level: no guess
         }
-        int myc = 0, mybad = 0;
-        const int cnt = min(kWave, n - base);
-        for (int j = 0; j < cnt; j++) {
-            const int ij = rl(idx, j);
-            const uint32_t fj = rlu(f, j);
-            const int lim = sat_add_i32(g.bad_sad, sat_mul_i32(g.bad_sad, c) / 16);
-            const int bj = (ij > 1 && (int)fj > lim) ? 1 : 0;
-            if (lane == j) { myc = c; mybad = bj; }
-            c += bj;
-        }
-        if (k < n) {
-            st.cpre[ch.base + idx] = myc;
-            if ((uint32_t)mybad != st.badf[ch.base + idx]) queue_block(st, qu, ci, ch, idx, s);
-        }
+        fs[k] = idx > 1 ? st.found[ch.base + idx] : 0u;
     }
-    if (lane == 0) st.minpos[ci] = 0xFFFFFFFFu;
+    int* wsum = sh + T;
+    int in = guess;
+    for (int it = 0; it <= T; it++) {  // exact after T iterations at the latest
+        const int d = scan_walk(fs, lo, hi, g.bad_sad, in) - in;
+        const int v = c0 + block_excl_scan(d, wsum);
+        const int changed = v != in;
+        in = v;
+        if (!__syncthreads_or(changed)) break;
+    }
+    int c = in;
+    for (int k = lo; k < hi; k++) {
+        const int pk = ch.order[k];
+        const int idx = (pk >> 16) * g.gw + (pk & 0xFFFF);
+        const int bj = (int)fs[k] > sat_add_i32(g.bad_sad, sat_mul_i32(g.bad_sad, c) / 16) ? 1 : 0;
+        st.cpre[ch.base + idx] = c;
+        if ((uint32_t)bj != st.badf[ch.base + idx]) queue_block(st, qu, ci, ch, idx, s);
+        c += bj;
+    }
+    if (tid == 0) st.minpos[ci] = 0xFFFFFFFFu;
 }
 
-__global__ void __launch_bounds__(kWave) sweep_scan_kernel(const Chain* __restrict__ chains, SweepState st, int s) {
-    scan_chain(chains, blockIdx.x, st, Queue{nullptr}, s);
+__global__ void __launch_bounds__(kScanThreads) sweep_scan_kernel(const Chain* __restrict__ chains, SweepState st,
+                                                                   int s) {
+    __shared__ int sh[kScanThreads + kScanThreads / kWave];
+    scan_chain(chains, blockIdx.x, st, Queue{nullptr}, s, sh);
 }
 
 // Evaluates block idx of chain ci in sweep s; when its vector (or score) changes, the
@@ -982,12 +1029,23 @@ __device__ __forceinline__ void sweep_item(const Chain* __restrict__ chains, int
     }
 }
 
-// Sweep 0: every block of every chain (grid: blocks x chains).
-__global__ void __launch_bounds__(kWave, 3) sweep0_kernel(const Chain* __restrict__ chains, SweepState st) {
-    const int ci = blockIdx.y, idx = blockIdx.x;
-    const LevelGeom* g = chains[ci].geom;
-    if (idx >= g->gw * g->gh) return;
-    sweep_item(chains, ci, idx, st, Queue{nullptr}, 0);
+// Sweep 0: every block of every chain, dispatched in visit order with the chains
+// interleaved (workgroup k * nchains + ci takes visit position k of chain ci).  Results
+// are stored in place as they are found, so a block dispatched after its
+// earlier-visited neighbours have finished already seeds from their searched vectors,
+// and its re-evaluation in sweep 1 (queued by their change) usually repeats the same
+// predictor winner and takes the memoised refinement.
+__global__ void __launch_bounds__(kWave, 3) sweep0_kernel(const Chain* __restrict__ chains, SweepState st,
+                                                          int nchains, long long total) {
+    for (long long u = blockIdx.x; u < total; u += gridDim.x) {
+        const int ci = (int)(u % nchains), k = (int)(u / nchains);
+        const LevelGeom* g = chains[ci].geom;
+        if (k >= g->gw * g->gh) continue;
+        const int pk = chains[ci].order[k];
+        const int bx = pk & 0xFFFF, by = pk >> 16;
+        if (bx >= g->gw || by >= g->gh) continue;
+        sweep_item(chains, ci, by * g->gw + bx, st, Queue{nullptr}, 0);
+    }
 }
 
 // Sweep s >= 1 over the blocks queued by sweep s - 1.
@@ -1031,7 +1089,9 @@ constexpr int kTailCap = 1 << 20;  // sweeps
 // (s0 + 1) & 1, i.e. sweep s0 + 1.
 __global__ void __launch_bounds__(kWave * kTailWaves) tail_kernel(const Chain* __restrict__ chains, SweepState st,
                                                                   int s0) {
+    static_assert(kWave * kTailWaves == kScanThreads, "the tail's workgroup rebuilds the count");
     __shared__ uint32_t cnt[2];
+    __shared__ int scan_sh[kScanThreads + kScanThreads / kWave];
     const int ci = blockIdx.x;
     const int wave = threadIdx.x / kWave;
     const uint32_t base = chains[ci].base;
@@ -1057,7 +1117,7 @@ __global__ void __launch_bounds__(kWave * kTailWaves) tail_kernel(const Chain* _
             sweep_item(chains, ci, idx, st, qu, s);
         }
         __syncthreads();
-        if (wave == 0) scan_chain(chains, ci, st, qu, s);
+        scan_chain(chains, ci, st, qu, s, scan_sh);
         __syncthreads();
         if (threadIdx.x == 0) cnt[p] = 0;
         __syncthreads();
@@ -1370,7 +1430,9 @@ hipError_t launch_sweep_init(const Chain* d_chains, int nchains, int max_n, cons
 }
 
 hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s) {
-    hipLaunchKernelGGL(sweep0_kernel, dim3(max_n, nchains), dim3(kWave), 0, s, d_chains, st);
+    const long long total = (long long)max_n * nchains;
+    const unsigned grid = (unsigned)std::min<long long>(total, 1 << 22);
+    hipLaunchKernelGGL(sweep0_kernel, dim3(grid), dim3(kWave), 0, s, d_chains, st, nchains, total);
     return hipGetLastError();
 }
 
@@ -1386,7 +1448,7 @@ hipError_t launch_tail(const Chain* d_chains, int nchains, const SweepState& st,
 }
 
 hipError_t launch_sweep_scan(const Chain* d_chains, int nchains, const SweepState& st, int sweep, hipStream_t s) {
-    hipLaunchKernelGGL(sweep_scan_kernel, dim3(nchains), dim3(kWave), 0, s, d_chains, st, sweep);
+    hipLaunchKernelGGL(sweep_scan_kernel, dim3(nchains), dim3(kScanThreads), 0, s, d_chains, st, sweep);
     return hipGetLastError();
 }
 

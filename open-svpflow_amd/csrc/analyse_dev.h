@@ -76,7 +76,8 @@ struct alignas(8) Memo {
 // Coarse levels with the bad-block pass also keep, per block, the pre-expansion cost
 // (`found`), whether the expansion ran (`badf`) and the running count of bad blocks
 // visited before it (`cpre`, rebuilt after every sweep from the first visit position
-// whose found/bad changed, `minpos` per chain).
+// whose found/bad changed, `minpos` per chain, with `fseq` as the rebuild's copy of
+// the costs in visit order).
 struct SweepState {
     unsigned long long* wl[2];  // items: chain index << 32 | block raster index
     uint32_t* flags[2];         // per block (chain base + raster index): queued for that parity
@@ -87,6 +88,7 @@ struct SweepState {
     uint32_t* minpos;           // per chain
     uint32_t* tail_count;       // per chain: list length handed to the per-chain tail
     Memo* memo;                 // per block
+    uint32_t* fseq;             // per block: pre-expansion costs in visit order (count rebuild)
     int max_sweeps;
 };
 

@@ -338,7 +338,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
     }
     // sweep state per level: flags x2 (one entry per block of every chain), worklists x2, counters
     size_t level_blocks[16] = {}, flags_off[16][2], wl_off[16][2], counts_off[16], found_off[16], badf_off[16],
-        cpre_off[16], minpos_off[16], tailc_off[16], memo_off[16];
+        cpre_off[16], minpos_off[16], tailc_off[16], memo_off[16], fseq_off[16];
     int level_max_n[16] = {};
     for (int l = 0; l < max_levels; l++) {
         size_t tot = 0;
@@ -362,6 +362,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
         minpos_off[l] = work.take(4 * (size_t)nchains + 4);
         tailc_off[l] = work.take(4 * (size_t)nchains + 4);
         memo_off[l] = work.take(sizeof(Memo) * (tot + 1));
+        fseq_off[l] = work.take(4 * tot + 4);
     }
     for (int l = 0; l < max_levels; l++) {
         size_t tot = 0;
@@ -623,6 +624,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
             st.minpos = (uint32_t*)(W + minpos_off[l]);
             st.tail_count = (uint32_t*)(W + tailc_off[l]);
             st.memo = (Memo*)(W + memo_off[l]);
+            st.fseq = (uint32_t*)(W + fseq_off[l]);
             st.max_sweeps = kMaxSweeps;
             SVPF_HIP(launch_sweep_init(dch, nch_step[l], level_max_n[l], st, ctx->stream));
             int rc2 = seq_step[l] ? SVPF_OK
