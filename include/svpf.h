@@ -67,8 +67,9 @@ int svpf_timer_stop(svpf_ctx* ctx, float* ms);
 int svpf_profile_enable(svpf_ctx* ctx, int on);
 int svpf_profile_read(svpf_ctx* ctx, const char* kernel, double* total_ms, int32_t* launches);
 /* Analyse sweep statistics of one pyramid level since profiling was enabled (no
- * counterpart in the reference): out4 = {level runs, sweeps launched, sweeps that had
- * work, block evaluations}. */
+ * counterpart in the reference; synchronizes while enabled): out4 = {level runs,
+ * global sweeps launched, sweeps of the per-chain tail (largest over the chains),
+ * block evaluations of the global sweeps}. */
 int svpf_analyse_stats(svpf_ctx* ctx, int32_t level, int64_t* out4);
 
 /* ------------------------------------------------------------------ Super */

@@ -1122,6 +1122,7 @@ __global__ void __launch_bounds__(kWave * kTailWaves) tail_kernel(const Chain* _
         if (threadIdx.x == 0) cnt[p] = 0;
         __syncthreads();
     }
+    if (threadIdx.x == 0 && s - s0 <= kTailCap) st.tail_count[ci] = (uint32_t)(s - s0 - 1);  // sweeps run
 }
 
 // ------------------------------------------------------------------ level transitions

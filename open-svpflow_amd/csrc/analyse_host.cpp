@@ -149,6 +149,14 @@ constexpr int kMaxSweeps = 4096;
 constexpr int kTailPerChain = 4;  // hand over to the per-chain tail below this many queued blocks per chain
 constexpr int kNotConverged = 1;
 
+// Block evaluations of the global sweeps 0..s-1 (profiling only; synchronizes).
+long long count_evaluations(svpf_ctx* ctx, const SweepState& st, int s, size_t total_blocks) {
+    if (hipMemcpy(ctx->h_count, st.counts, 4 * (size_t)s, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    long long ev = (long long)total_blocks;
+    for (int k = 1; k < s; k++) ev += ctx->h_count[k];
+    return ev;
+}
+
 // Parallel sweeps of one level until no block changes (each sweep evaluates the blocks
 // whose earlier-visited neighbours changed in the previous sweep).  The fixed point is
 // unique -- every block depends only on blocks visited before it -- and equals the
@@ -182,30 +190,26 @@ int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total
             SVPF_HIP(launch_tail(dch, nch, st, s, ctx->stream));
             SVPF_HIP(hipMemcpyAsync(ctx->h_count, st.tail_count, 4 * (size_t)nch, hipMemcpyDeviceToHost, ctx->stream));
             SVPF_HIP(hipStreamSynchronize(ctx->stream));
-            for (int c = 0; c < nch; c++)
+            uint32_t rounds = 0;
+            for (int c = 0; c < nch; c++) {
                 if (ctx->h_count[c] == 0xFFFFFFFFu) return kNotConverged;
+                rounds = std::max(rounds, ctx->h_count[c]);
+            }
             if (ctx->prof) {
                 long long* t = ctx->sweep_stats[std::min(level, 15)];
                 t[0] += 1;
                 t[1] += s;
-                t[2] += s;
+                t[2] += rounds;
+                t[3] += count_evaluations(ctx, st, s, total_blocks);
             }
             return SVPF_OK;
         }
         if (*ctx->h_count == 0) {
             if (ctx->prof) {  // statistics for the benchmark
-                SVPF_HIP(hipMemcpy(ctx->h_count, st.counts, 4 * (size_t)(s + 1), hipMemcpyDeviceToHost));
                 long long* t = ctx->sweep_stats[std::min(level, 15)];
                 t[0] += 1;
                 t[1] += s;
-                long long ev = (long long)total_blocks;
-                int last = 0;
-                for (int k = 1; k <= s; k++) {
-                    ev += ctx->h_count[k];
-                    if (ctx->h_count[k]) last = k;
-                }
-                t[2] += last + 1;
-                t[3] += ev;
+                t[3] += count_evaluations(ctx, st, s, total_blocks);
             }
             return SVPF_OK;
         }
