@@ -117,10 +117,22 @@ __device__ __forceinline__ int64_t rl64(int64_t v, int lane) {
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
+// Cross-lane moves without the LDS crossbar: DPP within rows of 16 lanes, the gfx950
+// permlane swaps across rows and halves.
+template <int Ctrl>
+__device__ __forceinline__ uint32_t dpp(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, Ctrl, 0xF, 0xF, false);
+}
+constexpr int kDppQuadXor1 = 0xB1;   // quad_perm [1, 0, 3, 2]
+constexpr int kDppQuadXor2 = 0x4E;   // quad_perm [2, 3, 0, 1]
+constexpr int kDppHalfMirror = 0x141;  // lane i <-> 7 - i within each 8 lanes
+constexpr int kDppRowRor8 = 0x128;     // lane i <- lane i ^ 8 within each 16 lanes
+
+// Sum over the 8 lanes of a group (every lane of the group receives it).
 __device__ __forceinline__ uint32_t xor_sum8(uint32_t v) {
-    v += __shfl_xor(v, 1);
-    v += __shfl_xor(v, 2);
-    v += __shfl_xor(v, 4);
+    v += dpp<kDppQuadXor1>(v);
+    v += dpp<kDppQuadXor2>(v);
+    v += dpp<kDppHalfMirror>(v);  // the two quads of the group hold their sums
     return v;
 }
 
@@ -218,6 +230,114 @@ __device__ __forceinline__ void group_cost(const Blk& b, int mvx, int mvy, bool 
     }
 }
 
+// ---- compile-time tile path of 4x4 and 8x8 blocks
+
+typedef __attribute__((address_space(1))) const uint8_t g_u8;
+typedef __attribute__((address_space(1))) const uint32_t __attribute__((aligned(1))) g_u32u;
+__device__ __forceinline__ uint32_t gld32(const g_u8* p) { return *reinterpret_cast<const g_u32u*>(p); }
+
+// Each lane of a group owns one 4x4 tile of the block -- parts 0-3 the luma tiles of
+// an 8x8 block (part 0 alone for 4x4), parts 4 and 5 the u and v tiles -- with its
+// source rows loaded once per block.  Candidates whose luma and chroma blocks lie
+// inside their levels and off the edge rule (the vectors of one rectangle, fixed per
+// block: the conditions of group_cost's interior path) take this path; the rest take
+// group_cost.
+struct TilePlan {
+    int on;                       // uniform
+    int xlo, xhi, ylo, yhi;       // uniform: the rectangle of vectors
+    int subpel, pel;              // uniform
+    int use, chroma;              // lane: owns a tile; the tile is chroma
+    int sh, subrows, rs, row0, col0;
+    const g_u8* base;
+    uint32_t src[4];
+};
+
+__device__ __forceinline__ void make_plan(const Blk& b, bool chroma, int part, TilePlan& P) {
+    P.on = 0;
+    const int N = b.bw;
+    if (b.bw != b.bh || (N != 4 && N != 8)) return;
+    const bool subpel = b.level == 0 && b.pel >= 2;
+    const int ipx = b.px - ((b.level > 0 && b.px + b.bw >= b.lw) ? 1 : 0);
+    const int ipy = b.py - ((b.level > 0 && b.py + b.bh >= b.lh) ? 1 : 0);
+    const int cw = N / 2;
+    const bool cuse = chroma && cw >= 4;
+    const int cpx = b.px / 2, cpy = b.py / 2;
+    if (ipx < 0 || ipy < 0 || ipx + N > b.lw || ipy + N > b.lh) return;
+    if (cuse && (cpx + cw > b.clw || cpy + cw > b.clh)) return;
+    const int sh = subpel ? 1 : 0, csh = subpel ? 2 : 1;
+    const int pel = b.pel;
+    const int b0 = -pel * b.px, b1 = -pel * b.py;
+    const int b2 = pel * (b.lw - b.px - b.bw) + 1, b3 = pel * (b.lh - b.py - b.bh) + 1;
+    // (m >> s) >= a  <=>  m >= a * 2^s;  (m >> s) <= a  <=>  m <= (a + 1) * 2^s - 1
+    int xlo = max(b0, -ipx * (1 << sh)), ylo = max(b1, -ipy * (1 << sh));
+    int xhi = (b.lw - N - ipx + 1) * (1 << sh) - 1, yhi = (b.lh - N - ipy + 1) * (1 << sh) - 1;
+    if (b2 > b0) xhi = min(xhi, b2 - 1);
+    if (b3 > b1) yhi = min(yhi, b3 - 1);
+    if (cuse) {
+        xlo = max(xlo, -cpx * (1 << csh));
+        ylo = max(ylo, -cpy * (1 << csh));
+        xhi = min(xhi, (b.clw - cw - cpx + 1) * (1 << csh) - 1);
+        yhi = min(yhi, (b.clh - cw - cpy + 1) * (1 << csh) - 1);
+    }
+    P.on = 1;
+    P.xlo = xlo; P.xhi = xhi; P.ylo = ylo; P.yhi = yhi;
+    P.subpel = subpel ? 1 : 0;
+    P.pel = pel;
+    const int nl = N == 8 ? 4 : 1;
+    P.use = part < nl + (cuse ? 2 : 0);
+    P.chroma = P.use && part >= nl;
+    const int pl = P.chroma ? part - nl + 1 : 0;
+    int srow, scol;
+    if (P.chroma) {
+        P.sh = csh;
+        P.subrows = subpel ? b.clh : 0;
+        P.row0 = b.c_off + cpy;
+        P.col0 = cpx;
+        srow = b.c_off + cpy;
+        scol = cpx;
+    } else {
+        const int tx = part & 1, ty = (part >> 1) & 1;
+        P.sh = sh;
+        P.subrows = b.lh;
+        P.row0 = b.y_off + ipy + 4 * ty;
+        P.col0 = ipx + 4 * tx;
+        srow = b.y_off + ipy + 4 * ty;
+        scol = ipx + 4 * tx;
+    }
+    P.rs = b.rs[pl];
+    P.base = (const g_u8*)b.rp[pl];
+    const g_u8* sp = (const g_u8*)b.sp[pl] + (srow * b.ss[pl] + scol);
+#pragma unroll
+    for (int k = 0; k < 4; k++) P.src[k] = P.use ? gld32(sp + k * b.ss[pl]) : 0u;
+}
+
+__device__ __forceinline__ bool plan_covers(const TilePlan& P, int mvx, int mvy) {
+    return P.on && mvx >= P.xlo && mvx <= P.xhi && mvy >= P.ylo && mvy <= P.yhi;
+}
+
+// The candidate's (luma, 4 x chroma) cost through the plan; every lane of the group
+// receives it (the same values as group_cost).
+__device__ __forceinline__ void plan_cost(const TilePlan& P, bool satd, int mvx, int mvy, uint32_t& L, uint32_t& C) {
+    uint32_t h = 0;
+    if (P.use) {
+        const int sub = P.subpel ? (mvy & 1) * P.pel + (mvx & 1) : 0;
+        const int row = P.row0 + sub * P.subrows + (mvy >> P.sh);
+        const int col = P.col0 + (mvx >> P.sh);
+        const g_u8* r = P.base + (row * P.rs + col);
+        uint32_t y[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) y[k] = gld32(r + k * P.rs);
+        if (satd) {
+            h = had4_rows_pk(P.src, y) >> 1;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) h = __builtin_amdgcn_sad_u8(P.src[k], y[k], h);
+        }
+    }
+    L = xor_sum8(P.chroma ? 0u : h);
+    C = 4u * xor_sum8(P.chroma ? h : 0u);
+}
+
 struct Best {
     int mvx, mvy;
     uint32_t sad;
@@ -237,20 +357,41 @@ __device__ __forceinline__ bool in_bounds(const Search& s, int x, int y) {
     return x >= s.bnd[0] && y >= s.bnd[1] && x < s.bnd[2] && y < s.bnd[3];
 }
 
-// Lowest (cost, index) across the groups of the wave (values are uniform per group).
+__device__ __forceinline__ void take_lower(int64_t& c, int& k, int64_t oc, int ok) {
+    if (oc < c || (oc == c && ok < k)) { c = oc; k = ok; }
+}
+
+__device__ __forceinline__ int64_t join64(uint32_t lo, uint32_t hi) { return (int64_t)(((uint64_t)hi << 32) | lo); }
+
+// Lowest (cost, index) across the groups of the wave (values are uniform per group;
+// every lane receives the result; needs the whole wave active).
 __device__ __forceinline__ void group_argmin(int64_t& c, int& k) {
-#pragma unroll
-    for (int off = kGroup; off < kWave; off <<= 1) {
-        const int64_t oc = __shfl_xor(c, off);
-        const int ok = __shfl_xor(k, off);
-        if (oc < c || (oc == c && ok < k)) { c = oc; k = ok; }
+    {  // groups g and g ^ 1
+        const uint32_t lo = dpp<kDppRowRor8>((uint32_t)c), hi = dpp<kDppRowRor8>((uint32_t)((uint64_t)c >> 32));
+        take_lower(c, k, join64(lo, hi), (int)dpp<kDppRowRor8>((uint32_t)k));
+    }
+    {  // rows r and r ^ 1: the swap leaves one row's values in each result
+        const auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)c, (uint32_t)c, false, false);
+        const auto hi = __builtin_amdgcn_permlane16_swap((uint32_t)((uint64_t)c >> 32),
+                                                         (uint32_t)((uint64_t)c >> 32), false, false);
+        const auto kk = __builtin_amdgcn_permlane16_swap((uint32_t)k, (uint32_t)k, false, false);
+        c = join64(lo[0], hi[0]);
+        k = (int)kk[0];
+        take_lower(c, k, join64(lo[1], hi[1]), (int)kk[1]);
+    }
+    {  // halves
+        const auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)c, (uint32_t)c, false, false);
+        const auto hi = __builtin_amdgcn_permlane32_swap((uint32_t)((uint64_t)c >> 32),
+                                                         (uint32_t)((uint64_t)c >> 32), false, false);
+        const auto kk = __builtin_amdgcn_permlane32_swap((uint32_t)k, (uint32_t)k, false, false);
+        c = join64(lo[0], hi[0]);
+        k = (int)kk[0];
+        take_lower(c, k, join64(lo[1], hi[1]), (int)kk[1]);
     }
 }
 
 __constant__ int kHex2[8][2] = {{-1, -2}, {-2, 0}, {-1, 2}, {1, 2}, {2, 0}, {1, -2}, {-1, -2}, {-2, 0}};
 __constant__ int kMod6m1[8] = {5, 0, 1, 2, 3, 4, 5, 0};
-__constant__ int kHex4[16][2] = {{-4, 2}, {-4, 1}, {-4, 0}, {-4, -1}, {-4, -2}, {4, -2}, {4, -1}, {4, 0},
-                                 {4, 1},  {4, 2},  {2, 3},  {0, 4},   {-2, 3},  {-2, -3}, {0, -4}, {2, -3}};
 
 // Candidate lists of the reference's search steps, in its evaluation order.
 enum PatKind : int {
@@ -274,47 +415,56 @@ __device__ __forceinline__ int pat_count(int kind, int r, int step) {
     return (2 * r + 1) * (2 * r + 1);  // kPatExh
 }
 
+// kHex2 and kHex4 as 4-bit fields (x + 2, y + 2 and x + 4, y + 4), so a lane's entry
+// is a shift of a constant rather than a memory read.
+constexpr uint32_t kHex2X = 0x01343101u, kHex2Y = 0x20024420u;
+constexpr uint64_t kHex4X = 0x6422468888800000ull, kHex4Y = 0x1017876543223456ull;
+
 __device__ __forceinline__ void pat_gen(const Pattern& p, int k, int& x, int& y) {
     const int cx = p.cx, cy = p.cy, r = p.r;
+    const unsigned uk = (unsigned)k;
     if (p.kind == kPatExpand) {
-        const int m = (2 * r - 1) / p.step;  // values of i in -r+step, -r+2step, ... < r
+        const int m = (2 * r - 1) / p.step;  // values of i in -r+step, -r+2step, ... < r (uniform)
         if (k < 2 * m) {
-            x = cx - r + p.step * (1 + k / 2);
+            x = cx - r + p.step * (1 + (int)(uk >> 1));
             y = (k & 1) ? cy + r : cy - r;
         } else if (k < 4 * m) {
-            const int j = k - 2 * m;
-            y = cy - r + p.step * (1 + j / 2);
+            const unsigned j = uk - 2 * m;
+            y = cy - r + p.step * (1 + (int)(j >> 1));
             x = (j & 1) ? cx + r : cx - r;
         } else {
             const int c = k - 4 * m;  // (-r,-r), (-r,+r), (+r,-r), (+r,+r)
             x = (c < 2) ? cx - r : cx + r;
             y = (c & 1) ? cy + r : cy - r;
         }
-    } else if (p.kind == kPatHexFirst) {
-        x = cx + kHex2[k + 1][0];
-        y = cy + kHex2[k + 1][1];
-    } else if (p.kind == kPatHexStep) {  // p.step holds odir
-        x = cx + kHex2[p.step + k][0];
-        y = cy + kHex2[p.step + k][1];
+    } else if (p.kind == kPatHexFirst || p.kind == kPatHexStep) {  // kHex2[k + 1], or kHex2[odir + k] (p.step = odir)
+        const unsigned e = p.kind == kPatHexFirst ? uk + 1 : (unsigned)p.step + uk;
+        x = cx + (int)((kHex2X >> (4 * e)) & 15u) - 2;
+        y = cy + (int)((kHex2Y >> (4 * e)) & 15u) - 2;
     } else if (p.kind == kPatUmh) {
         const int nd = r / 2;
         if (k < 2 * nd) {
-            const int d = 1 + 2 * (k / 2);
+            const int d = 1 + 2 * (int)(uk >> 1);
             x = (k & 1) ? cx + d : cx - d;
             y = cy;
         } else if (k < 4 * nd) {
-            const int j = k - 2 * nd, d = 1 + 2 * (j / 2);
+            const unsigned j = uk - 2 * nd;
+            const int d = 1 + 2 * (int)(j >> 1);
             x = cx;
             y = (j & 1) ? cy + d : cy - d;
         } else {
-            const int j = k - 4 * nd, sc = 1 + j / 16, h = j % 16;
-            x = cx + kHex4[h][0] * sc;
-            y = cy + kHex4[h][1] * sc;
+            const unsigned j = uk - 4 * nd, h = j & 15u;
+            const int sc = 1 + (int)(j >> 4);
+            x = cx + ((int)((kHex4X >> (4 * h)) & 15u) - 4) * sc;
+            y = cy + ((int)((kHex4Y >> (4 * h)) & 15u) - 4) * sc;
         }
-    } else {  // kPatExh
-        const int w = 2 * r + 1;
-        x = cx - r + k % w;
-        y = cy - r + k / w;
+    } else {  // kPatExh: k = row * w + column, the quotient by a multiply (k, w < 2^16)
+        const unsigned w = 2u * (unsigned)r + 1u;
+        unsigned q = w == 1u ? uk : __umulhi(uk, 0xFFFFFFFFu / w + 1u);
+        if (q * w > uk) q--;
+        if (uk - q * w >= w) q++;
+        x = cx - r + (int)(uk - q * w);
+        y = cy - r + (int)q;
     }
 }
 
@@ -540,6 +690,8 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
     Pattern P;
     bool pchroma;
     int step;
+    TilePlan tp;  // for the block at its own position (the edge predictor's measure uses group_cost)
+    tp.on = 0;
     if (edge && !smallest) {
         b.px = epx; b.py = epy;
         lact = lane == 0; lx = pred.dx; ly = pred.dy; lpen = 0;
@@ -547,6 +699,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
         pchroma = true;
         step = kStEdgeDone;
     } else {
+        make_plan(b, !smallest, part, tp);
         set_pred_list();
         P = Pattern{kPatPred, 0, 0, 0, 0, 4 + nseeds};
         pchroma = !smallest;
@@ -563,22 +716,32 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
         // ---------------- the evaluation site: kPerPass candidates per pass, one group each
         int win = -1;
         {
+            // the pattern is wave-uniform: keep it in scalar registers so the pattern
+            // branches below are scalar branches
+            P = uniform_load(&P);
             const bool list = P.kind == kPatPred;
             for (int base = 0; base < P.n; base += kPerPass) {
                 const int k = base + (lane >> 3);
-                const int kl = min(k, kWave - 1);
-                const int tx = __shfl(lx, kl), ty = __shfl(ly, kl), tact = __shfl(lact, kl), tpen = __shfl(lpen, kl);
-                int mx = 0, my = 0;
+                int mx = 0, my = 0, tpen = 0;
                 bool valid = k < P.n;
                 if (list) {
-                    mx = tx; my = ty;
+                    // every lane takes part in the shuffles: a lane past the list end
+                    // still holds a candidate another group reads
+                    const int kl = min(k, kWave - 1);
+                    mx = __shfl(lx, kl);
+                    my = __shfl(ly, kl);
+                    tpen = __shfl(lpen, kl);
+                    const int tact = __shfl(lact, kl);
                     valid = valid && tact != 0;
                 } else if (valid) {
                     pat_gen(P, k, mx, my);
                     valid = in_bounds(cur, mx, my);
                 }
                 uint32_t L = 0, C = 0;
-                if (valid) group_cost(b, mx, my, pchroma, part, L, C);
+                if (valid) {
+                    if (plan_covers(tp, mx, my)) plan_cost(tp, b.satd, mx, my, L, C);
+                    else group_cost(b, mx, my, pchroma, part, L, C);
+                }
                 const uint32_t sad = sat_add(L, C);
                 int64_t cost = INT64_MAX, motion = INT64_MAX;
                 if (valid) {
@@ -637,6 +800,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                 case kStEdgeDone:
                     pred.score = rlu(lsad, 0);
                     b.px = px; b.py = py;
+                    make_plan(b, !smallest, part, tp);
                     set_pred_list();
                     best = Best{0, 0, 0xFFFFFFFFu, INT64_MAX};
                     P = Pattern{kPatPred, 0, 0, 0, 0, 4 + nseeds};
