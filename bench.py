@@ -27,6 +27,8 @@ import sys
 import threading
 import time
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "open-svpflow_amd"))
 
@@ -151,6 +153,53 @@ def cpu_baseline(frames, pairs: int):
     return nout, dt, 2 * len(sub)
 
 
+SERVE_RES = ((854, 480), (1280, 720), (1920, 1080), (2560, 1440))
+SERVE_WEIGHTS = (0.2, 0.3, 0.4, 0.1)
+
+
+def serving_run(ctx, seconds: float, pixel_rate_budget: float, seed: int = 5000, arrivals_per_s: float = 40.0,
+                budget_gib: float = 48.0):
+    """Config-5 serving: Poisson arrivals of streams (480p/720p/1080p/1440p, 60-600
+    frames, 2x at 30 fps or 2.5x at 24 fps) over `seconds`, each delivering frames in
+    real time; admission by an HBM budget and the throughput budget; the config-2
+    Analyse / SmoothFps settings.  Content: a device pool of 6 frames per size of the
+    layered generator, played back and forth so that motion stays continuous."""
+    import serve
+
+    rng = np.random.default_rng(seed)
+    pools = {}
+    for i, (w, h) in enumerate(SERVE_RES):
+        frames = synth.clip(w, h, 6, seed=seed + i, vmax=16.0, nobj=(6, 12), rot_deg=2.0)
+        dev = []
+        for f in frames:
+            d = svpf.DeviceFrame.alloc(ctx, (w, w // 2, w // 2), (h, h // 2, h // 2))
+            d.upload(f)
+            dev.append(d)
+        pools[(w, h)] = dev
+
+    def source(dev, off):
+        order = list(range(len(dev))) + list(range(len(dev) - 2, 0, -1))
+        return lambda i: dev[order[(off + i) % len(order)]]
+
+    specs, t, sid = [], 0.0, 0
+    while True:
+        t += rng.exponential(1.0 / arrivals_per_s)
+        if t >= seconds:
+            break
+        w, h = SERVE_RES[rng.choice(len(SERVE_RES), p=SERVE_WEIGHTS)]
+        two = rng.random() < 0.5
+        specs.append(serve.StreamSpec(sid, w, h, int(rng.integers(60, 601)), (2, 1) if two else (5, 2),
+                                      (30, 1) if two else (24, 1), t, source(pools[(w, h)], int(rng.integers(10)))))
+        sid += 1
+    srv = serve.Server(ctx, budget_bytes=int(budget_gib * (1 << 30)), max_wait=1.0,
+                       pixel_rate_budget=pixel_rate_budget)
+    stats = srv.run(specs, duration=seconds)
+    srv.close()
+    stats.update({"arrivals": len(specs), "arrival_rate_per_s": arrivals_per_s, "hbm_budget_gib": budget_gib,
+                  "pixel_rate_budget": round(pixel_rate_budget), "window_s": seconds})
+    return stats
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,7 +207,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--frames", type=int, default=120)
     ap.add_argument("--plain-frames", type=int, default=3, help="mini-clip length of the per-frame path sample")
-    ap.add_argument("--cpu-pairs", type=int, default=4, help="frame pairs of the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-pairs", type=int, default=8, help="frame pairs of the CPU baseline sample (0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=15.0,
+                    help="length of the config-5 serving run that gives the p50/p99 frame latency (0: skip)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run 4 device-to-device copies of MIB MiB each (the known-size reference for "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE scaling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the motion path shards by clip / frame range with no collective "
@@ -168,6 +222,16 @@ def main():
     print(f"[bench] generating {args.frames} frames 1920x1080", file=sys.stderr, flush=True)
     frames = synth.config_clip(2, frames=args.frames)
     ctx = svpf.Context(0)
+    if args.copy_calibration > 0:
+        nb = args.copy_calibration << 20
+        a, b = ctx.alloc(nb), ctx.alloc(nb)
+        ctx.memset(a, 1, nb)
+        for _ in range(4):
+            ctx.copy(b, a, nb)
+        ctx.sync()
+        ctx.free(a)
+        ctx.free(b)
+        print(f"[bench] copy calibration: 4 x {nb} bytes device to device", file=sys.stderr, flush=True)
     clip = Clip(ctx, frames)
     print(f"[bench] warmup {args.warmup}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
@@ -194,11 +258,13 @@ def main():
     value = clip.nout * args.steps / wall
 
     # plain per-frame path on a mini-clip
-    print(f"[bench] plain per-frame path, {args.plain_frames} frames", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    plain_out = clip.plain_pass(args.plain_frames)
-    plain_dt = time.perf_counter() - t0
-    plain_fps = plain_out / plain_dt
+    plain_out, plain_dt, plain_fps = 0, 0.0, 0.0
+    if args.plain_frames > 1:
+        print(f"[bench] plain per-frame path, {args.plain_frames} frames", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        plain_out = clip.plain_pass(args.plain_frames)
+        plain_dt = time.perf_counter() - t0
+        plain_fps = plain_out / plain_dt
 
     search_ms, search_n = prof["search"]
     sb = search_bytes_per_step(clip) * args.steps
@@ -210,6 +276,14 @@ def main():
     fb = clip.w * clip.h * 1.5
     sbytes = (fb + clip.super_rows * clip.w * 1.5) * clip.n * args.steps
     super_gbs = sbytes / (super_ms / 1000.0) / 1e9 if super_ms > 0 else 0.0
+
+    serving = None
+    if args.serve_seconds > 0:
+        # throughput budget: 40% of the source pixel rate the batched step sustained (serving
+        # steps are small batches, whose level searches are bound by latency, not throughput)
+        src_rate = clip.n * clip.w * clip.h * args.steps / wall
+        print(f"[bench] serving run, {args.serve_seconds:.0f} s of arrivals", file=sys.stderr, flush=True)
+        serving = serving_run(ctx, args.serve_seconds, 0.4 * src_rate)
 
     cpu = None
     if args.cpu_pairs > 0:
@@ -242,7 +316,8 @@ def main():
                   "output_frames": plain_out, "seconds": round(plain_dt, 3)},
         "batched_over_plain": round(value / plain_fps, 2) if plain_fps > 0 else None,
         "gpu_ms_per_step_events": round(gpu_ms / args.steps, 2),
-        "roofline": {"bound": "hbm", "kernel": "search_level_kernel (Analyse block search)",
+        "roofline": {"bound": "hbm", "kernel": "Analyse level search (sweep0/sweep/tail kernels of one pyramid "
+                                                "level; integer-VALU bound)",
                      "achieved": round(search_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": search_gbs / HBM_PEAK_GBS, "traffic": None,
                      "launches": search_n, "avg_launch_ms": round(search_ms / max(search_n, 1), 3),
@@ -255,7 +330,8 @@ def main():
                            "avg_launch_ms": round(super_ms / max(super_n, 1), 3)},
         "stage_ms_per_step": {k: round(v[0] / args.steps, 2) for k, v in prof.items()},
         "analyse_sweeps": {f"level{lv}": v for lv, v in sweeps.items()},
-        "p99_frame_latency_ms": None,
+        "p99_frame_latency_ms": serving["p99_ms"] if serving else None,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

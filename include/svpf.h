@@ -54,6 +54,8 @@ int svpf_device_alloc(svpf_ctx* ctx, size_t bytes, void** d_out);
 int svpf_device_free(svpf_ctx* ctx, void* d_ptr);
 int svpf_copy_h2d(svpf_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
 int svpf_copy_d2h(svpf_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
+/* Device-to-device copy on the context's stream (asynchronous). */
+int svpf_copy_d2d(svpf_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
 int svpf_memset_d(svpf_ctx* ctx, void* d_dst, int value, size_t bytes);
 /* Event timing on the context's stream: start, then stop returns elapsed ms. */
 int svpf_timer_start(svpf_ctx* ctx);
@@ -175,6 +177,12 @@ int svpf_smooth_set_sources(svpf_smooth* s, int32_t first, int32_t count, const 
  * `algo`; the call then synchronizes. */
 int svpf_smooth_render(svpf_smooth* s, int32_t k0, int32_t count, const svpf_frame* d_out, int32_t* h_decision);
 int32_t svpf_smooth_output_frames(const svpf_smooth* s);
+/* Unbinds source frames first..first+count-1 and frees the device state the session
+ * derived from them (decoded vectors, motion planes, masks), for callers that stream
+ * a long clip through a window of frames (no counterpart in the reference, whose
+ * frame cache is bounded by VapourSynth).  Synchronizes if any state was held.  A
+ * frame is bound again with svpf_smooth_set_sources before a render that needs it. */
+int svpf_smooth_release(svpf_smooth* s, int32_t first, int32_t count);
 
 #ifdef __cplusplus
 }

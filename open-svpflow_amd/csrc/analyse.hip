@@ -31,6 +31,12 @@ using namespace cost;
 // wavefronts per workgroup).
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
 
+// One of three values by a per-lane index, selected as values: a conditional over
+// struct members is an lvalue, which the compiler turns into a selected address and a
+// load, placing the whole struct in scratch memory.
+template <class T>
+__device__ __forceinline__ T pick3(int i, T a, T b, T c) { return i == 0 ? a : (i == 1 ? b : c); }
+
 // Packed 16-bit SATD of one 4x4 tile (same value as had4_rows): the differences and
 // the first three butterfly stages run two columns per instruction; the last stage
 // uses |a + b| + |a - b| = 2 max(|a|, |b|), so it needs no butterflies.  Every
@@ -169,12 +175,12 @@ __device__ __forceinline__ void group_cost(const Blk& b, int mvx, int mvy, bool 
     if (fast) {
         const int rrow = b.y_off + sub * b.lh;
         const int cref = b.c_off + ((b.level == 0 && pel > 1) ? sub * b.clh : 0);
-        const uint8_t* ls = b.sp[0] + (int64_t)(b.y_off + ipy) * b.ss[0] + ipx;
-        const uint8_t* lr = b.rp[0] + (int64_t)(rrow + fry) * b.rs[0] + frx;
-        const uint8_t* us = b.sp[1] + (int64_t)(b.c_off + cpy) * b.ss[1] + cpx;
-        const uint8_t* ur = b.rp[1] + (int64_t)(cref + cry) * b.rs[1] + crx;
-        const uint8_t* vs = b.sp[2] + (int64_t)(b.c_off + cpy) * b.ss[2] + cpx;
-        const uint8_t* vr = b.rp[2] + (int64_t)(cref + cry) * b.rs[2] + crx;
+        const uint8_t* ls = b.sp0 + (int64_t)(b.y_off + ipy) * b.ss0 + ipx;
+        const uint8_t* lr = b.rp0 + (int64_t)(rrow + fry) * b.rs0 + frx;
+        const uint8_t* us = b.sp1 + (int64_t)(b.c_off + cpy) * b.ss1 + cpx;
+        const uint8_t* ur = b.rp1 + (int64_t)(cref + cry) * b.rs1 + crx;
+        const uint8_t* vs = b.sp2 + (int64_t)(b.c_off + cpy) * b.ss2 + cpx;
+        const uint8_t* vr = b.rp2 + (int64_t)(cref + cry) * b.rs2 + crx;
         if (sq) {  // luma 4x4 tiles, then u tiles, then v tiles (chroma SATD whenever luma is square SATD)
             const int lt = b.bw >> 2, ct = cw >> 2;
             const int nl = lt * lt, nc = cuse ? ct * ct : 0;
@@ -185,8 +191,8 @@ __device__ __forceinline__ void group_cost(const Blk& b, int mvx, int mvy, bool 
                 const int ty = t / per, tx = t - ty * per;
                 const uint8_t* sp = plane == 0 ? ls : (plane == 1 ? us : vs);
                 const uint8_t* rp = plane == 0 ? lr : (plane == 1 ? ur : vr);
-                const int ss = plane == 0 ? b.ss[0] : (plane == 1 ? b.ss[1] : b.ss[2]);
-                const int rs = plane == 0 ? b.rs[0] : (plane == 1 ? b.rs[1] : b.rs[2]);
+                const int ss = pick3(plane, b.ss0, b.ss1, b.ss2);
+                const int rs = pick3(plane, b.rs0, b.rs1, b.rs2);
                 uint32_t x[4], y[4];
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
@@ -203,9 +209,9 @@ __device__ __forceinline__ void group_cost(const Blk& b, int mvx, int mvy, bool 
                 const int r = plane == 0 ? u : (plane == 1 ? u - nl : u - nl - nc);
                 const uint8_t* sp = plane == 0 ? ls : (plane == 1 ? us : vs);
                 const uint8_t* rp = plane == 0 ? lr : (plane == 1 ? ur : vr);
-                const int ss = plane == 0 ? b.ss[0] : (plane == 1 ? b.ss[1] : b.ss[2]);
-                const int rs = plane == 0 ? b.rs[0] : (plane == 1 ? b.rs[1] : b.rs[2]);
-                const int words = (plane == 0 ? b.bw : cw) >> 2;
+                const int ss = pick3(plane, b.ss0, b.ss1, b.ss2);
+                const int rs = pick3(plane, b.rs0, b.rs1, b.rs2);
+                const int words = (plane == 0 ? (int)b.bw : cw) >> 2;
                 uint32_t acc = 0;
                 for (int w = 0; w < words; w++)
                     acc = __builtin_amdgcn_sad_u8(ld32(sp + (int64_t)r * ss + 4 * w), ld32(rp + (int64_t)r * rs + 4 * w), acc);
@@ -304,11 +310,12 @@ __device__ __forceinline__ void make_plan(const Blk& b, bool chroma, int part, T
         srow = b.y_off + ipy + 4 * ty;
         scol = ipx + 4 * tx;
     }
-    P.rs = b.rs[pl];
-    P.base = (const g_u8*)b.rp[pl];
-    const g_u8* sp = (const g_u8*)b.sp[pl] + (srow * b.ss[pl] + scol);
+    P.rs = pick3(pl, b.rs0, b.rs1, b.rs2);
+    P.base = (const g_u8*)pick3(pl, b.rp0, b.rp1, b.rp2);
+    const int ss = pick3(pl, b.ss0, b.ss1, b.ss2);
+    const g_u8* sp = (const g_u8*)pick3(pl, b.sp0, b.sp1, b.sp2) + (srow * ss + scol);
 #pragma unroll
-    for (int k = 0; k < 4; k++) P.src[k] = P.use ? gld32(sp + k * b.ss[pl]) : 0u;
+    for (int k = 0; k < 4; k++) P.src[k] = P.use ? gld32(sp + k * ss) : 0u;
 }
 
 __device__ __forceinline__ bool plan_covers(const TilePlan& P, int mvx, int mvy) {
@@ -489,7 +496,7 @@ __device__ __attribute__((always_inline)) uint8_t luma_dc(const Blk& b) {
     const int cols = min(b.bw, max(b.lw - b.px, 0));
     for (int i = lane; i < rows * cols; i += kWave) {
         const int r = i / cols, c = i % cols;
-        sum += (uint32_t)rd(b.sp[0], b.sl[0], (int64_t)(b.y_off + b.py + r) * b.ss[0] + b.px + c, 0);
+        sum += (uint32_t)rd(b.sp0, b.sl0, (int64_t)(b.y_off + b.py + r) * b.ss0 + b.px + c, 0);
         cnt++;
     }
 #pragma unroll
@@ -548,18 +555,13 @@ __device__ __forceinline__ T uniform_load(const T* p) {
 }
 __device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// A result written earlier by this wave (per lane; the lanes may read different blocks).
-__device__ __forceinline__ DVec load_vec(const DVec* p) {
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    return *p;
-}
-
 __device__ __forceinline__ void init_blk(const Chain& ch, const LevelGeom& g, Blk& b) {
-    for (int i = 0; i < 3; i++) {
-        b.sp[i] = ch.src.p[i]; b.rp[i] = ch.ref.p[i];
-        b.ss[i] = ch.src.stride[i]; b.rs[i] = ch.ref.stride[i];
-        b.sl[i] = (int)ch.src.len[i]; b.rl[i] = (int)ch.ref.len[i];
-    }
+    b.sp0 = ch.src.p[0]; b.sp1 = ch.src.p[1]; b.sp2 = ch.src.p[2];
+    b.rp0 = ch.ref.p[0]; b.rp1 = ch.ref.p[1]; b.rp2 = ch.ref.p[2];
+    b.ss0 = ch.src.stride[0]; b.ss1 = ch.src.stride[1]; b.ss2 = ch.src.stride[2];
+    b.rs0 = ch.ref.stride[0]; b.rs1 = ch.ref.stride[1]; b.rs2 = ch.ref.stride[2];
+    b.sl0 = (int)ch.src.len[0]; b.sl1 = (int)ch.src.len[1]; b.sl2 = (int)ch.src.len[2];
+    b.rl0 = (int)ch.ref.len[0]; b.rl1 = (int)ch.ref.len[1]; b.rl2 = (int)ch.ref.len[2];
     b.level = g.level; b.lw = g.lw; b.lh = g.lh; b.y_off = g.y_off; b.c_off = g.c_off;
     b.clw = g.clw; b.clh = g.clh; b.pel = g.pel; b.satd = g.satd != 0;
     b.px = 0; b.py = 0; b.bw = g.bw; b.bh = g.bh;
@@ -594,7 +596,8 @@ enum Step : int {
 // order.  Returns the block's vector (uniform across the wave), *bad = whether the
 // coarse-level bad-block expansion ran and *found_out = the cost before it.
 __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g, Blk& b, int idx,
-                                             int bad_count, bool* bad, uint32_t* found_out, Memo* memo) {
+                                             int bad_count, bool* bad, uint32_t* found_out, Memo* memo,
+                                             SeedRec* rec, bool* unchanged) {
     const int lane = lane_id();
     const int part = lane & (kGroup - 1);
     const int gx = ch.global ? uniform_int(ch.global[0]) : 0, gy = ch.global ? uniform_int(ch.global[1]) : 0;
@@ -626,19 +629,32 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
     int radius = g.radius;
     if (level > 0 && g.adaptive) {
         const int ax = px - ((px + bw >= g.lw) ? 1 : 0), ay = py - ((py + bh >= g.lh) ? 1 : 0);
-        uint32_t act = plane_activity(b.sp[0], b.ss[0], g.y_off + ay, ax, bw, bh, b.satd) +
-                       plane_activity(b.sp[1], b.ss[1], g.c_off + py / 2, px / 2, bw / 2, bh / 2, b.satd) +
-                       plane_activity(b.sp[2], b.ss[2], g.c_off + py / 2, px / 2, bw / 2, bh / 2, b.satd);
+        uint32_t act = plane_activity(b.sp0, b.ss0, g.y_off + ay, ax, bw, bh, b.satd) +
+                       plane_activity(b.sp1, b.ss1, g.c_off + py / 2, px / 2, bw / 2, bh / 2, b.satd) +
+                       plane_activity(b.sp2, b.ss2, g.c_off + py / 2, px / 2, bw / 2, bh / 2, b.satd);
         radius = act < 2 ? 0 : ((radius * (int)min(255u * act / 150u, 255u)) >> 8) + 1;
     }
     // seeds: vectors of the 3x3 neighbours visited earlier, row-major (:574-584)
     int nseed_lane = 0;
     int sdx = 0, sdy = 0;
+    uint32_t seed_word = 0;
     if (lane < 9) {
         const int nx = bx - 1 + lane % 3, ny = by - 1 + lane / 3;
         if (nx >= 0 && ny >= 0 && nx < gw && ny < gh && ch.opos[ny * gw + nx] < me) {
-            const DVec nb = load_vec(&ch.res[ny * gw + nx]);
-            nseed_lane = 1; sdx = nb.dx; sdy = nb.dy;
+            // device-scope load: the neighbour may have been stored in this launch by
+            // another CU (its vector is the first word of the record)
+            seed_word = __hip_atomic_load(reinterpret_cast<const uint32_t*>(&ch.res[ny * gw + nx]),
+                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            nseed_lane = 1; sdx = (int16_t)(seed_word & 0xFFFFu); sdy = (int16_t)(seed_word >> 16);
+        }
+    }
+    // the same neighbours and bad-block count as the last evaluation: the same result
+    if (edge) rec = nullptr;
+    if (rec) {
+        const bool differs = lane < 9 && nseed_lane && rec->seed[lane] != seed_word;
+        if (__ballot(differs) == 0 && uniform_int((int)rec->valid) != 0 && uniform_int(rec->bad_count) == bad_count) {
+            *unchanged = true;
+            return DVec{};
         }
     }
     const uint64_t smask = __ballot(nseed_lane);
@@ -982,6 +998,13 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
     }
     *bad = is_bad;
     *found_out = found;
+    if (rec) {
+        if (lane < 9) rec->seed[lane] = seed_word;
+        if (lane == 0) {
+            rec->bad_count = bad_count;
+            rec->valid = 1;
+        }
+    }
     DVec o;
     o.dx = (int16_t)best.mvx;
     o.dy = (int16_t)best.mvy;
@@ -1008,7 +1031,7 @@ __global__ void __launch_bounds__(kWave, 3) search_level_kernel(const Chain* __r
         const int idx = by * g.gw + bx;
         bool bad;
         uint32_t found;
-        const DVec o = search_block(ch, g, b, idx, bad_count, &bad, &found, nullptr);
+        const DVec o = search_block(ch, g, b, idx, bad_count, &bad, &found, nullptr, nullptr, nullptr);
         bad_count += bad ? 1 : 0;
         if (threadIdx.x == 0) ch.res[idx] = o;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -1032,6 +1055,7 @@ __global__ void sweep_init_kernel(const Chain* __restrict__ chains, SweepState s
         st.badf[ch.base + i] = 0;
         st.cpre[ch.base + i] = 1 << 30;  // sweep 0 runs no bad-block pass: the limit saturates
         st.memo[ch.base + i].valid = 0;
+        st.seeds[ch.base + i].valid = 0;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st.minpos[blockIdx.y] = 0;  // the first rebuild covers the whole order
@@ -1166,8 +1190,13 @@ __device__ __forceinline__ void sweep_item(const Chain* __restrict__ chains, int
     init_blk(ch, g, b);
     bool bad;
     uint32_t found;
+    bool unchanged = false;
     const DVec o = search_block(ch, g, b, idx, uniform_int(st.cpre[ch.base + idx]), &bad, &found,
-                                st.memo + ch.base + idx);
+                                st.memo + ch.base + idx, st.seeds + ch.base + idx, &unchanged);
+    if (unchanged) {  // nothing it reads changed since its last evaluation
+        if (lane_id() == 0) atomicAdd(&st.counts[st.max_sweeps + 1], 1u);  // skip count (statistics)
+        return;
+    }
     const DVec old = uniform_load(&ch.res[idx]);
     const bool moved = o.dx != old.dx || o.dy != old.dy || o.score != old.score;
     const uint32_t oldf = (uint32_t)uniform_int((int)st.found[ch.base + idx]);
@@ -1193,19 +1222,25 @@ __device__ __forceinline__ void sweep_item(const Chain* __restrict__ chains, int
     }
 }
 
-// Sweep 0: every block of every chain, dispatched in visit order with the chains
-// interleaved (workgroup k * nchains + ci takes visit position k of chain ci).  Results
-// are stored in place as they are found, so a block dispatched after its
-// earlier-visited neighbours have finished already seeds from their searched vectors,
-// and its re-evaluation in sweep 1 (queued by their change) usually repeats the same
-// predictor winner and takes the memoised refinement.
+// Sweep 0: every block of every chain, dispatched in visit order.  Each chain's visit
+// order is cut into `nseg` segments, and the (chain, segment) slots v = ci * nseg + seg
+// are interleaved: item u = k * nslots + v takes position k of slot v.  With nslots a
+// multiple of 8 (and the grid too), the hardware's round-robin of workgroups over the
+// 8 XCDs sends every item of a slot to the same XCD, whose L2 then holds the results
+// its earlier items stored in place.  So a block dispatched after its earlier-visited
+// neighbours have finished already seeds from their searched vectors, and its
+// re-evaluation in sweep 1 (queued by their change) usually repeats the same predictor
+// winner and takes the memoised refinement.
 __global__ void __launch_bounds__(kWave, 3) sweep0_kernel(const Chain* __restrict__ chains, SweepState st,
-                                                          int nchains, long long total) {
+                                                          int nslots, int nseg, long long total) {
     for (long long u = blockIdx.x; u < total; u += gridDim.x) {
-        const int ci = (int)(u % nchains), k = (int)(u / nchains);
+        const int v = (int)(u % nslots), k = (int)(u / nslots);
+        const int ci = v / nseg, seg = v - ci * nseg;
         const LevelGeom* g = chains[ci].geom;
-        if (k >= g->gw * g->gh) continue;
-        const int pk = chains[ci].order[k];
+        const int n = g->gw * g->gh, len = (n + nseg - 1) / nseg;
+        const int pos = seg * len + k;
+        if (k >= len || pos >= n) continue;
+        const int pk = chains[ci].order[pos];
         const int bx = pk & 0xFFFF, by = pk >> 16;
         if (bx >= g->gw || by >= g->gh) continue;
         sweep_item(chains, ci, by * g->gw + bx, st, Queue{nullptr}, 0);
@@ -1595,9 +1630,24 @@ hipError_t launch_sweep_init(const Chain* d_chains, int nchains, int max_n, cons
 }
 
 hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s) {
-    const long long total = (long long)max_n * nchains;
+    // segments per chain: a multiple of q so that the slot count is a multiple of the 8
+    // XCDs, about 1024 slots (few items of a slot in flight at once; more segments put
+    // more blocks at a segment start, where they read neighbours of the previous
+    // segment before those are searched), and no segment much shorter than 2048 blocks
+    int g8 = 8;
+    for (int a = nchains % 8; a != 0;) {
+        const int t = g8 % a;
+        g8 = a;
+        a = t;
+    }
+    const int q = 8 / g8;
+    int nseg = q * ((1024 + nchains * q - 1) / (nchains * q));
+    const int cap = std::max(q, max_n / 2048 / q * q);
+    nseg = std::min(nseg, cap);
+    const int nslots = nchains * nseg;
+    const long long total = (long long)nslots * ((max_n + nseg - 1) / nseg);
     const unsigned grid = (unsigned)std::min<long long>(total, 1 << 22);
-    hipLaunchKernelGGL(sweep0_kernel, dim3(grid), dim3(kWave), 0, s, d_chains, st, nchains, total);
+    hipLaunchKernelGGL(sweep0_kernel, dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
     return hipGetLastError();
 }
 

@@ -90,10 +90,12 @@ SVPF_UNROLL
 
 // The block being searched (uniform across the wave).
 struct Blk {
-    const uint8_t* sp[3];
-    const uint8_t* rp[3];
-    int ss[3], rs[3];
-    int sl[3], rl[3];  // plane byte lengths (< 2^31)
+    // one field per plane, no arrays: a plane chosen per lane must compile to selects
+    // (an array indexed that way would place the descriptor in scratch memory)
+    const uint8_t *sp0, *sp1, *sp2;
+    const uint8_t *rp0, *rp1, *rp2;
+    int ss0, ss1, ss2, rs0, rs1, rs2;
+    int sl0, sl1, sl2, rl0, rl1, rl2;  // plane byte lengths (< 2^31)
     int level, lw, lh, y_off, c_off, clw, clh, pel;
     int px, py, bw, bh;
     bool satd;
@@ -139,9 +141,9 @@ SVPF_HD inline void cand_cost_full(const Blk& b, int mvx, int mvy, bool chroma, 
     const bool sq = b.satd && pow2_4_32(b.bw) && pow2_4_32(b.bh) && b.bw == b.bh;
     const bool subpel = b.level == 0 && pel >= 2;
     const int sub = subpel ? (mvy & 1) * pel + (mvx & 1) : 0;
-    const uint8_t *S0 = b.sp[0], *R0 = b.rp[0], *S1 = b.sp[1], *R1 = b.rp[1], *S2 = b.sp[2], *R2 = b.rp[2];
-    const int sl0 = b.sl[0], rl0 = b.rl[0], sl1 = b.sl[1], rl1 = b.rl[1], sl2 = b.sl[2], rl2 = b.rl[2];
-    const int ss0 = b.ss[0], rs0 = b.rs[0], ss1 = b.ss[1], rs1 = b.rs[1], ss2 = b.ss[2], rs2 = b.rs[2];
+    const uint8_t *S0 = b.sp0, *R0 = b.rp0, *S1 = b.sp1, *R1 = b.rp1, *S2 = b.sp2, *R2 = b.rp2;
+    const int sl0 = b.sl0, rl0 = b.rl0, sl1 = b.sl1, rl1 = b.rl1, sl2 = b.sl2, rl2 = b.rl2;
+    const int ss0 = b.ss0, rs0 = b.rs0, ss1 = b.ss1, rs1 = b.rs1, ss2 = b.ss2, rs2 = b.rs2;
     const int rrow = b.y_off + sub * b.lh;
     if (ox != 0 || oy != 0) {
         // ---- edge path (edge_shift_origins :1638-1676 + block_cost_edge_lc): every

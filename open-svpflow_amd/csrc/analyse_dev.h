@@ -69,6 +69,18 @@ struct alignas(8) Memo {
     int64_t rcost;    // its cost (the bad-block expansion compares against it)
 };
 
+// What a block's last evaluation read from the level: the vector of each of its 3x3
+// neighbours visited before it (slot 4, the block itself, unused) and the running
+// bad-block count.  The other inputs of a block off the edge row and column are fixed
+// for the level, so an evaluation whose neighbours and count read the same values
+// again would reproduce the stored result; sweeps skip it.
+struct alignas(16) SeedRec {
+    uint32_t seed[9];  // (dy << 16) | (uint16_t)dx of present neighbours
+    int32_t bad_count;
+    uint32_t valid;
+    uint32_t pad;
+};
+
 // Worklists of the parallel sweeps of one level (all chains together).  Sweep s
 // processes wl[s & 1][0 .. counts[s]) whose members are flagged in flags[s & 1]; a
 // block whose result changes queues its later-visited 3x3 neighbours for sweep s + 1.
@@ -88,6 +100,7 @@ struct SweepState {
     uint32_t* minpos;           // per chain
     uint32_t* tail_count;       // per chain: list length handed to the per-chain tail
     Memo* memo;                 // per block
+    SeedRec* seeds;             // per block
     uint32_t* fseq;             // per block: pre-expansion costs in visit order (count rebuild)
     int max_sweeps;
 };

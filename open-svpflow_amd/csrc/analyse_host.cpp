@@ -149,12 +149,15 @@ constexpr int kMaxSweeps = 4096;
 constexpr int kTailPerChain = 4;  // hand over to the per-chain tail below this many queued blocks per chain
 constexpr int kNotConverged = 1;
 
-// Block evaluations of the global sweeps 0..s-1 (profiling only; synchronizes).
+// Blocks evaluated by the global sweeps 0..s-1, less the evaluations skipped because
+// nothing the block reads had changed (counts[max_sweeps + 1], the tail's skips
+// included).  Profiling only; synchronizes.
 long long count_evaluations(svpf_ctx* ctx, const SweepState& st, int s, size_t total_blocks) {
-    if (hipMemcpy(ctx->h_count, st.counts, 4 * (size_t)s, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    if (hipMemcpy(ctx->h_count, st.counts, 4 * (size_t)(st.max_sweeps + 2), hipMemcpyDeviceToHost) != hipSuccess)
+        return -1;
     long long ev = (long long)total_blocks;
     for (int k = 1; k < s; k++) ev += ctx->h_count[k];
-    return ev;
+    return ev - (long long)ctx->h_count[st.max_sweeps + 1];
 }
 
 // Parallel sweeps of one level until no block changes (each sweep evaluates the blocks
@@ -342,7 +345,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
     }
     // sweep state per level: flags x2 (one entry per block of every chain), worklists x2, counters
     size_t level_blocks[16] = {}, flags_off[16][2], wl_off[16][2], counts_off[16], found_off[16], badf_off[16],
-        cpre_off[16], minpos_off[16], tailc_off[16], memo_off[16], fseq_off[16];
+        cpre_off[16], minpos_off[16], tailc_off[16], memo_off[16], fseq_off[16], seed_off[16];
     int level_max_n[16] = {};
     for (int l = 0; l < max_levels; l++) {
         size_t tot = 0;
@@ -367,6 +370,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
         tailc_off[l] = work.take(4 * (size_t)nchains + 4);
         memo_off[l] = work.take(sizeof(Memo) * (tot + 1));
         fseq_off[l] = work.take(4 * tot + 4);
+        seed_off[l] = work.take(sizeof(SeedRec) * (tot + 1));
     }
     for (int l = 0; l < max_levels; l++) {
         size_t tot = 0;
@@ -629,6 +633,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
             st.tail_count = (uint32_t*)(W + tailc_off[l]);
             st.memo = (Memo*)(W + memo_off[l]);
             st.fseq = (uint32_t*)(W + fseq_off[l]);
+            st.seeds = (SeedRec*)(W + seed_off[l]);
             st.max_sweeps = kMaxSweeps;
             SVPF_HIP(launch_sweep_init(dch, nch_step[l], level_max_n[l], st, ctx->stream));
             int rc2 = seq_step[l] ? SVPF_OK

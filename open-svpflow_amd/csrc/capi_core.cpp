@@ -212,6 +212,12 @@ int svpf_copy_d2h(svpf_ctx* ctx, void* h_dst, const void* d_src, size_t bytes) {
     return SVPF_OK;
 }
 
+int svpf_copy_d2d(svpf_ctx* ctx, void* d_dst, const void* d_src, size_t bytes) {
+    if (!ctx) return fail(SVPF_ERR_ARG, "null context");
+    SVPF_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return SVPF_OK;
+}
+
 int svpf_memset_d(svpf_ctx* ctx, void* d_dst, int value, size_t bytes) {
     if (!ctx) return fail(SVPF_ERR_ARG, "null context");
     SVPF_HIP(hipMemsetAsync(d_dst, value, bytes, ctx->stream));

@@ -582,4 +582,22 @@ int svpf_smooth_render(svpf_smooth* s, int32_t k0, int32_t count, const svpf_fra
 
 int32_t svpf_smooth_output_frames(const svpf_smooth* s) { return s ? s->out_frames : 0; }
 
+int svpf_smooth_release(svpf_smooth* s, int32_t first, int32_t count) {
+    if (!s || first < 0 || count < 0 || first + count > s->nframes)
+        return fail(SVPF_ERR_ARG, "svpf_smooth_release: bad argument");
+    bool any = false;
+    for (int i = 0; i < count; i++) any = any || s->prep[(size_t)(first + i)] != nullptr;
+    if (any) SVPF_HIP(hipStreamSynchronize(s->ctx->stream));  // queued work may still read them
+    for (int i = 0; i < count; i++) {
+        const size_t f = (size_t)(first + i);
+        if (s->prep[f]) (void)hipFree(s->prep[f]);
+        s->prep[f] = nullptr;
+        s->prepped[f] = 0;
+        s->vec[f] = nullptr;
+        s->src[f] = svpf_frame{};
+        s->sup[f] = svpf_frame{};
+    }
+    return SVPF_OK;
+}
+
 }  // extern "C"

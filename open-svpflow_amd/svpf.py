@@ -59,6 +59,7 @@ def lib() -> C.CDLL:
         L.svpf_copy_h2d.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
         L.svpf_copy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
         L.svpf_memset_d.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_size_t]
+        L.svpf_copy_d2d.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
         L.svpf_super_params_from_opt.argtypes = [C.c_char_p, C.c_size_t, C.c_int32, C.c_int32,
                                                  C.POINTER(SuperParams), C.POINTER(C.c_int64), C.c_char_p,
                                                  C.c_size_t]
@@ -140,6 +141,10 @@ class Context:
         out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype=dtype)
         check(lib().svpf_copy_d2h(self.h, out.ctypes.data, dptr, out.nbytes))
         return out
+
+    def copy(self, dst: int, src: int, nbytes: int) -> None:
+        """Device-to-device copy on the context's stream."""
+        check(lib().svpf_copy_d2d(self.h, dst, src, nbytes))
 
     def memset(self, dptr: int, value: int, nbytes: int) -> None:
         check(lib().svpf_memset_d(self.h, dptr, value, nbytes))
@@ -310,6 +315,7 @@ def _bind_smooth(L):
     L.svpf_smooth_render.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(Frame), C.POINTER(C.c_int32)]
     L.svpf_smooth_output_frames.argtypes = [C.c_void_p]
     L.svpf_smooth_output_frames.restype = C.c_int32
+    L.svpf_smooth_release.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
 
 
 _BINDERS.append(_bind_smooth)
@@ -357,6 +363,10 @@ class Smooth:
         U = (Frame * n)(*[f.desc() for f in supers]) if supers is not None else None
         V = (C.c_void_p * n)(*d_vectors)
         check(lib().svpf_smooth_set_sources(self.h, first, n, S, U, V))
+
+    def release(self, first: int, count: int) -> None:
+        """Frees the device state derived from source frames first..first+count-1."""
+        check(lib().svpf_smooth_release(self.h, first, count))
 
     def render(self, k0: int, outs, decisions: bool = False):
         n = len(outs)

@@ -13,12 +13,12 @@ extern "C" int svpf_host_cand_cost(int level, int lw, int lh, int y_off, int c_o
                                    int py, int bw, int bh, int mvx, int mvy, int chroma, uint32_t* L, uint32_t* C) {
     Blk b;
     std::memset(&b, 0, sizeof b);
-    b.sp[0] = cy; b.sp[1] = cu; b.sp[2] = cv;
-    b.rp[0] = ry; b.rp[1] = ru; b.rp[2] = rv;
-    b.ss[0] = cys; b.ss[1] = cus; b.ss[2] = cvs;
-    b.rs[0] = rys; b.rs[1] = rus; b.rs[2] = rvs;
-    b.sl[0] = cyl; b.sl[1] = cul; b.sl[2] = cvl;
-    b.rl[0] = ryl; b.rl[1] = rul; b.rl[2] = rvl;
+    b.sp0 = cy; b.sp1 = cu; b.sp2 = cv;
+    b.rp0 = ry; b.rp1 = ru; b.rp2 = rv;
+    b.ss0 = cys; b.ss1 = cus; b.ss2 = cvs;
+    b.rs0 = rys; b.rs1 = rus; b.rs2 = rvs;
+    b.sl0 = cyl; b.sl1 = cul; b.sl2 = cvl;
+    b.rl0 = ryl; b.rl1 = rul; b.rl2 = rvl;
     b.level = level; b.lw = lw; b.lh = lh; b.y_off = y_off; b.c_off = c_off;
     b.clw = lw / 2; b.clh = lh / 2; b.pel = pel; b.satd = satd != 0;
     b.px = px; b.py = py; b.bw = bw; b.bh = bh;
