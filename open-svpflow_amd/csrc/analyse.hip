@@ -111,7 +111,7 @@ __device__ __forceinline__ bool block_fast_any(int n, int m, const uint8_t* s, i
 
 // ------------------------------------------------------------------ cooperative cost
 
-constexpr int kGroup = 8;                 // lanes per candidate
+constexpr int kGroup = 4;                 // lanes per candidate
 constexpr int kPerPass = kWave / kGroup;  // candidates per pass
 
 // Wave-uniform reads of one lane's value (scalar results keep the search state in SGPRs).
@@ -131,18 +131,17 @@ __device__ __forceinline__ uint32_t dpp(uint32_t v) {
 }
 constexpr int kDppQuadXor1 = 0xB1;   // quad_perm [1, 0, 3, 2]
 constexpr int kDppQuadXor2 = 0x4E;   // quad_perm [2, 3, 0, 1]
-constexpr int kDppHalfMirror = 0x141;  // lane i <-> 7 - i within each 8 lanes
+constexpr int kDppRowRor4 = 0x124;     // lane i <- lane (i + 4) mod 16 within each 16 lanes
 constexpr int kDppRowRor8 = 0x128;     // lane i <- lane i ^ 8 within each 16 lanes
 
-// Sum over the 8 lanes of a group (every lane of the group receives it).
-__device__ __forceinline__ uint32_t xor_sum8(uint32_t v) {
+// Sum over the 4 lanes of a group (a quad; every lane of the group receives it).
+__device__ __forceinline__ uint32_t group_sum(uint32_t v) {
     v += dpp<kDppQuadXor1>(v);
     v += dpp<kDppQuadXor2>(v);
-    v += dpp<kDppHalfMirror>(v);  // the two quads of the group hold their sums
     return v;
 }
 
-// One candidate's cost computed by the 8 lanes of a group (part = lane & 7); every
+// One candidate's cost computed by the 4 lanes of a group (part = lane & 3); every
 // lane of the group receives (L, C).  Blocks whose luma and chroma lie inside their
 // levels (the common case) split into units -- 4x4 tiles for SATD, rows for SAD --
 // spread over the group; any other path is computed whole by part 0 (cand_cost_full).
@@ -223,9 +222,9 @@ __device__ __forceinline__ void group_cost(const Blk& b, int mvx, int mvy, bool 
     } else if (part == 0) {
         cand_cost_full(b, mvx, mvy, chroma, a0, a1);
     }
-    a0 = xor_sum8(a0);
-    a1 = xor_sum8(a1);
-    a2 = xor_sum8(a2);
+    a0 = group_sum(a0);
+    a1 = group_sum(a1);
+    a2 = group_sum(a2);
     if (fast) {
         Lout = sq ? (a0 >> 1) : a0;
         const uint32_t U = b.satd ? (a1 >> 1) : a1, V = b.satd ? (a2 >> 1) : a2;
@@ -242,20 +241,19 @@ typedef __attribute__((address_space(1))) const uint8_t g_u8;
 typedef __attribute__((address_space(1))) const uint32_t __attribute__((aligned(1))) g_u32u;
 __device__ __forceinline__ uint32_t gld32(const g_u8* p) { return *reinterpret_cast<const g_u32u*>(p); }
 
-// Each lane of a group owns one 4x4 tile of the block -- parts 0-3 the luma tiles of
-// an 8x8 block (part 0 alone for 4x4), parts 4 and 5 the u and v tiles -- with its
-// source rows loaded once per block.  Candidates whose luma and chroma blocks lie
-// inside their levels and off the edge rule (the vectors of one rectangle, fixed per
-// block: the conditions of group_cost's interior path) take this path; the rest take
-// group_cost.
+// Each lane of a group (part = lane % 4) owns the luma 4x4 tile `part` of an 8x8 block
+// (part 0 alone for 4x4); parts 0 and 1 also own the u and v tiles.  Their source rows
+// are loaded once per block.  Candidates whose luma and chroma blocks lie inside their
+// levels and off the edge rule (the vectors of one rectangle, fixed per block: the
+// conditions of group_cost's interior path) take this path; the rest take group_cost.
 struct TilePlan {
     int on;                       // uniform
     int xlo, xhi, ylo, yhi;       // uniform: the rectangle of vectors
-    int subpel, pel;              // uniform
-    int use, chroma;              // lane: owns a tile; the tile is chroma
-    int sh, subrows, rs, row0, col0;
-    const g_u8* base;
-    uint32_t src[4];
+    int subpel, pel, lh, clh;     // uniform
+    int lrow0, lcol0, luse;       // lane: luma tile
+    int crow0, ccol0, cuse, crs;  // lane: chroma tile
+    const g_u8* cbase;
+    uint32_t lsrc[4], csrc[4];
 };
 
 __device__ __forceinline__ void make_plan(const Blk& b, bool chroma, int part, TilePlan& P) {
@@ -289,60 +287,61 @@ __device__ __forceinline__ void make_plan(const Blk& b, bool chroma, int part, T
     P.xlo = xlo; P.xhi = xhi; P.ylo = ylo; P.yhi = yhi;
     P.subpel = subpel ? 1 : 0;
     P.pel = pel;
-    const int nl = N == 8 ? 4 : 1;
-    P.use = part < nl + (cuse ? 2 : 0);
-    P.chroma = P.use && part >= nl;
-    const int pl = P.chroma ? part - nl + 1 : 0;
-    int srow, scol;
-    if (P.chroma) {
-        P.sh = csh;
-        P.subrows = subpel ? b.clh : 0;
-        P.row0 = b.c_off + cpy;
-        P.col0 = cpx;
-        srow = b.c_off + cpy;
-        scol = cpx;
-    } else {
-        const int tx = part & 1, ty = (part >> 1) & 1;
-        P.sh = sh;
-        P.subrows = b.lh;
-        P.row0 = b.y_off + ipy + 4 * ty;
-        P.col0 = ipx + 4 * tx;
-        srow = b.y_off + ipy + 4 * ty;
-        scol = ipx + 4 * tx;
-    }
-    P.rs = pick3(pl, b.rs0, b.rs1, b.rs2);
-    P.base = (const g_u8*)pick3(pl, b.rp0, b.rp1, b.rp2);
-    const int ss = pick3(pl, b.ss0, b.ss1, b.ss2);
-    const g_u8* sp = (const g_u8*)pick3(pl, b.sp0, b.sp1, b.sp2) + (srow * ss + scol);
+    P.lh = b.lh;
+    P.clh = subpel ? b.clh : 0;
+    const int tx = part & 1, ty = (part >> 1) & 1;
+    P.luse = N == 8 || part == 0;
+    P.lrow0 = b.y_off + ipy + 4 * ty;
+    P.lcol0 = ipx + 4 * tx;
+    P.cuse = cuse && part < 2;
+    P.crow0 = b.c_off + cpy;
+    P.ccol0 = cpx;
+    P.crs = part == 0 ? b.rs1 : b.rs2;
+    P.cbase = (const g_u8*)(part == 0 ? b.rp1 : b.rp2);
+    const int css = part == 0 ? b.ss1 : b.ss2;
+    const g_u8* ls = (const g_u8*)b.sp0 + (P.lrow0 * b.ss0 + P.lcol0);
+    const g_u8* cs = (const g_u8*)(part == 0 ? b.sp1 : b.sp2) + (P.crow0 * css + P.ccol0);
 #pragma unroll
-    for (int k = 0; k < 4; k++) P.src[k] = P.use ? gld32(sp + k * ss) : 0u;
+    for (int k = 0; k < 4; k++) {
+        P.lsrc[k] = P.luse ? gld32(ls + k * b.ss0) : 0u;
+        P.csrc[k] = P.cuse ? gld32(cs + k * css) : 0u;
+    }
 }
 
 __device__ __forceinline__ bool plan_covers(const TilePlan& P, int mvx, int mvy) {
     return P.on && mvx >= P.xlo && mvx <= P.xhi && mvy >= P.ylo && mvy <= P.yhi;
 }
 
+// Cost of one 4x4 tile against the reference rows at r (stride rs).
+__device__ __forceinline__ uint32_t tile_cost(const uint32_t src[4], const g_u8* r, int rs, bool satd) {
+    uint32_t y[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) y[k] = gld32(r + k * rs);
+    if (satd) return had4_rows_pk(src, y) >> 1;
+    uint32_t h = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) h = __builtin_amdgcn_sad_u8(src[k], y[k], h);
+    return h;
+}
+
 // The candidate's (luma, 4 x chroma) cost through the plan; every lane of the group
 // receives it (the same values as group_cost).
-__device__ __forceinline__ void plan_cost(const TilePlan& P, bool satd, int mvx, int mvy, uint32_t& L, uint32_t& C) {
-    uint32_t h = 0;
-    if (P.use) {
-        const int sub = P.subpel ? (mvy & 1) * P.pel + (mvx & 1) : 0;
-        const int row = P.row0 + sub * P.subrows + (mvy >> P.sh);
-        const int col = P.col0 + (mvx >> P.sh);
-        const g_u8* r = P.base + (row * P.rs + col);
-        uint32_t y[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) y[k] = gld32(r + k * P.rs);
-        if (satd) {
-            h = had4_rows_pk(P.src, y) >> 1;
-        } else {
-#pragma unroll
-            for (int k = 0; k < 4; k++) h = __builtin_amdgcn_sad_u8(P.src[k], y[k], h);
-        }
+__device__ __forceinline__ void plan_cost(const TilePlan& P, const Blk& b, int mvx, int mvy, uint32_t& L,
+                                          uint32_t& C) {
+    const int sub = P.subpel ? (mvy & 1) * P.pel + (mvx & 1) : 0;
+    uint32_t hl = 0, hc = 0;
+    if (P.luse) {
+        const int sh = P.subpel;  // luma vectors are in half pels when sub-pel
+        const int row = P.lrow0 + sub * P.lh + (mvy >> sh), col = P.lcol0 + (mvx >> sh);
+        hl = tile_cost(P.lsrc, (const g_u8*)b.rp0 + (row * b.rs0 + col), b.rs0, b.satd);
     }
-    L = xor_sum8(P.chroma ? 0u : h);
-    C = 4u * xor_sum8(P.chroma ? h : 0u);
+    if (P.cuse) {
+        const int csh = P.subpel + 1;
+        const int row = P.crow0 + sub * P.clh + (mvy >> csh), col = P.ccol0 + (mvx >> csh);
+        hc = tile_cost(P.csrc, P.cbase + (row * P.crs + col), P.crs, b.satd);
+    }
+    L = group_sum(hl);
+    C = 4u * group_sum(hc);
 }
 
 struct Best {
@@ -373,7 +372,11 @@ __device__ __forceinline__ int64_t join64(uint32_t lo, uint32_t hi) { return (in
 // Lowest (cost, index) across the groups of the wave (values are uniform per group;
 // every lane receives the result; needs the whole wave active).
 __device__ __forceinline__ void group_argmin(int64_t& c, int& k) {
-    {  // groups g and g ^ 1
+    {  // the 4 groups of a row of 16 lanes: rotations by 4, then by 8
+        const uint32_t lo = dpp<kDppRowRor4>((uint32_t)c), hi = dpp<kDppRowRor4>((uint32_t)((uint64_t)c >> 32));
+        take_lower(c, k, join64(lo, hi), (int)dpp<kDppRowRor4>((uint32_t)k));
+    }
+    {
         const uint32_t lo = dpp<kDppRowRor8>((uint32_t)c), hi = dpp<kDppRowRor8>((uint32_t)((uint64_t)c >> 32));
         take_lower(c, k, join64(lo, hi), (int)dpp<kDppRowRor8>((uint32_t)k));
     }
@@ -737,7 +740,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
             P = uniform_load(&P);
             const bool list = P.kind == kPatPred;
             for (int base = 0; base < P.n; base += kPerPass) {
-                const int k = base + (lane >> 3);
+                const int k = base + lane / kGroup;
                 int mx = 0, my = 0, tpen = 0;
                 bool valid = k < P.n;
                 if (list) {
@@ -755,7 +758,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                 }
                 uint32_t L = 0, C = 0;
                 if (valid) {
-                    if (plan_covers(tp, mx, my)) plan_cost(tp, b.satd, mx, my, L, C);
+                    if (plan_covers(tp, mx, my)) plan_cost(tp, b, mx, my, L, C);
                     else group_cost(b, mx, my, pchroma, part, L, C);
                 }
                 const uint32_t sad = sat_add(L, C);
@@ -1231,7 +1234,7 @@ __device__ __forceinline__ void sweep_item(const Chain* __restrict__ chains, int
 // neighbours have finished already seeds from their searched vectors, and its
 // re-evaluation in sweep 1 (queued by their change) usually repeats the same predictor
 // winner and takes the memoised refinement.
-__global__ void __launch_bounds__(kWave, 3) sweep0_kernel(const Chain* __restrict__ chains, SweepState st,
+__global__ void __launch_bounds__(kWave, 2) sweep0_kernel(const Chain* __restrict__ chains, SweepState st,
                                                           int nslots, int nseg, long long total) {
     for (long long u = blockIdx.x; u < total; u += gridDim.x) {
         const int v = (int)(u % nslots), k = (int)(u / nslots);
@@ -1248,7 +1251,7 @@ __global__ void __launch_bounds__(kWave, 3) sweep0_kernel(const Chain* __restric
 }
 
 // Sweep s >= 1 over the blocks queued by sweep s - 1.
-__global__ void __launch_bounds__(kWave, 3) sweep_kernel(const Chain* __restrict__ chains, SweepState st, int s) {
+__global__ void __launch_bounds__(kWave, 2) sweep_kernel(const Chain* __restrict__ chains, SweepState st, int s) {
     const uint32_t n = (uint32_t)uniform_int((int)__atomic_load_n(&st.counts[s], __ATOMIC_RELAXED));
     const int p = s & 1;
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
