@@ -1214,7 +1214,9 @@ __device__ __forceinline__ void sweep_item(const Chain* __restrict__ chains, int
             if (g.level > 0 && g.bad_sad > 0) atomicMin(&st.minpos[ci], (uint32_t)me);
         }
     }
-    if (!moved) return;
+    // sweep 0 queues nothing: validate_kernel compares every block's seed record with
+    // its neighbours once the sweep is complete
+    if (!moved || s == 0) return;
     if (lane < 9 && lane != 4) {
         const int bx = idx % g.gw, by = idx / g.gw;
         const int nx = bx - 1 + lane % 3, ny = by - 1 + lane / 3;
@@ -1231,9 +1233,8 @@ __device__ __forceinline__ void sweep_item(const Chain* __restrict__ chains, int
 // multiple of 8 (and the grid too), the hardware's round-robin of workgroups over the
 // 8 XCDs sends every item of a slot to the same XCD, whose L2 then holds the results
 // its earlier items stored in place.  So a block dispatched after its earlier-visited
-// neighbours have finished already seeds from their searched vectors, and its
-// re-evaluation in sweep 1 (queued by their change) usually repeats the same predictor
-// winner and takes the memoised refinement.
+// neighbours have finished already seeds from their searched vectors, and
+// validate_kernel finds its seed record up to date.
 __global__ void __launch_bounds__(kWave, 2) sweep0_kernel(const Chain* __restrict__ chains, SweepState st,
                                                           int nslots, int nseg, long long total) {
     for (long long u = blockIdx.x; u < total; u += gridDim.x) {
@@ -1247,6 +1248,37 @@ __global__ void __launch_bounds__(kWave, 2) sweep0_kernel(const Chain* __restric
         const int bx = pk & 0xFFFF, by = pk >> 16;
         if (bx >= g->gw || by >= g->gh) continue;
         sweep_item(chains, ci, by * g->gw + bx, st, Queue{nullptr}, 0);
+    }
+}
+
+// After sweep 0: queues for sweep 1 every block that must be evaluated again -- one
+// whose recorded neighbour vectors differ from the neighbours' results now (every
+// other input of a block off the edge row and column is fixed for the level; a changed
+// bad-block count is handled by the count rebuild), and every edge block (no record).
+// One thread per block, blocks in raster order.
+__global__ void validate_kernel(const Chain* __restrict__ chains, SweepState st, int max_n, long long total) {
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int ci = (int)(t / max_n), idx = (int)(t - (long long)ci * max_n);
+        const Chain& ch = chains[ci];
+        const LevelGeom& g = *ch.geom;
+        const int gw = g.gw, gh = g.gh;
+        if (idx >= gw * gh) continue;
+        const int bx = idx % gw, by = idx / gw;
+        bool stale = g.level > 0 && (bx == gw - 1 || by == gh - 1);
+        if (!stale) {
+            const SeedRec& r = st.seeds[ch.base + idx];
+            stale = r.valid == 0;
+            const int me = ch.opos[idx];
+            for (int q = 0; q < 9 && !stale; q++) {
+                const int nx = bx - 1 + q % 3, ny = by - 1 + q / 3;
+                if (q == 4 || nx < 0 || ny < 0 || nx >= gw || ny >= gh) continue;
+                const int n = ny * gw + nx;
+                if (ch.opos[n] >= me) continue;
+                stale = *reinterpret_cast<const uint32_t*>(&ch.res[n]) != r.seed[q];
+            }
+        }
+        if (stale) queue_block(st, Queue{}, ci, ch, idx, 0);
     }
 }
 
@@ -1662,6 +1694,13 @@ hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, i
 hipError_t launch_tail(const Chain* d_chains, int nchains, const SweepState& st, int sweep, hipStream_t s) {
     hipLaunchKernelGGL(tail_split_kernel, dim3(64), dim3(256), 0, s, d_chains, st, sweep);
     hipLaunchKernelGGL(tail_kernel, dim3(nchains), dim3(kWave * kTailWaves), 0, s, d_chains, st, sweep);
+    return hipGetLastError();
+}
+
+hipError_t launch_validate(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s) {
+    const long long total = (long long)nchains * max_n;
+    const unsigned grid = (unsigned)std::min<long long>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(validate_kernel, dim3(grid), dim3(256), 0, s, d_chains, st, max_n, total);
     return hipGetLastError();
 }
 
