@@ -755,7 +755,14 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                 } else if (valid) {
                     pat_gen(P, k, mx, my);
                     valid = in_bounds(cur, mx, my);
+                    if (valid && !cur.serial) {
+                        // cost >= motion (pnew >= -256): a candidate whose motion penalty
+                        // reaches the best so far cannot win, as in exact_check (:1122-1155)
+                        const int64_t dx = cur.pmx - mx, dy = cur.pmy - my;
+                        valid = (((int64_t)cur.lambda * (dx * dx + dy * dy)) >> 8) < best.cost;
+                    }
                 }
+                if (!list && __ballot(valid) == 0) continue;  // no candidate of this pass can win
                 uint32_t L = 0, C = 0;
                 if (valid) {
                     if (plan_covers(tp, mx, my)) plan_cost(tp, b, mx, my, L, C);
