@@ -279,11 +279,11 @@ def main():
 
     serving = None
     if args.serve_seconds > 0:
-        # throughput budget: 40% of the source pixel rate the batched step sustained (serving
+        # throughput budget: 25% of the source pixel rate the batched step sustained (serving
         # steps are small batches, whose level searches are bound by latency, not throughput)
         src_rate = clip.n * clip.w * clip.h * args.steps / wall
         print(f"[bench] serving run, {args.serve_seconds:.0f} s of arrivals", file=sys.stderr, flush=True)
-        serving = serving_run(ctx, args.serve_seconds, 0.4 * src_rate)
+        serving = serving_run(ctx, args.serve_seconds, 0.25 * src_rate)
 
     cpu = None
     if args.cpu_pairs > 0:
