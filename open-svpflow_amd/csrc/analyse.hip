@@ -250,6 +250,9 @@ struct TilePlan {
     int on;                       // uniform
     int xlo, xhi, ylo, yhi;       // uniform: the rectangle of vectors
     int subpel, pel, lh, clh;     // uniform
+    const g_u8* lbase;            // uniform: reference luma plane
+    int lrs, satd;                // uniform: its stride; SATD (else SAD)
+    int dc, n;                    // uniform: the tiles are the block's own pixels; block size
     int lrow0, lcol0, luse;       // lane: luma tile
     int crow0, ccol0, cuse, crs;  // lane: chroma tile
     const g_u8* cbase;
@@ -285,6 +288,11 @@ __device__ __forceinline__ void make_plan(const Blk& b, bool chroma, int part, T
     }
     P.on = 1;
     P.xlo = xlo; P.xhi = xhi; P.ylo = ylo; P.yhi = yhi;
+    P.lbase = (const g_u8*)b.rp0;
+    P.lrs = b.rs0;
+    P.satd = b.satd ? 1 : 0;
+    P.dc = ipx == b.px && ipy == b.py;
+    P.n = N;
     P.subpel = subpel ? 1 : 0;
     P.pel = pel;
     P.lh = b.lh;
@@ -326,19 +334,18 @@ __device__ __forceinline__ uint32_t tile_cost(const uint32_t src[4], const g_u8*
 
 // The candidate's (luma, 4 x chroma) cost through the plan; every lane of the group
 // receives it (the same values as group_cost).
-__device__ __forceinline__ void plan_cost(const TilePlan& P, const Blk& b, int mvx, int mvy, uint32_t& L,
-                                          uint32_t& C) {
+__device__ __forceinline__ void plan_cost(const TilePlan& P, int mvx, int mvy, uint32_t& L, uint32_t& C) {
     const int sub = P.subpel ? (mvy & 1) * P.pel + (mvx & 1) : 0;
     uint32_t hl = 0, hc = 0;
     if (P.luse) {
         const int sh = P.subpel;  // luma vectors are in half pels when sub-pel
         const int row = P.lrow0 + sub * P.lh + (mvy >> sh), col = P.lcol0 + (mvx >> sh);
-        hl = tile_cost(P.lsrc, (const g_u8*)b.rp0 + (row * b.rs0 + col), b.rs0, b.satd);
+        hl = tile_cost(P.lsrc, P.lbase + (row * P.lrs + col), P.lrs, P.satd != 0);
     }
     if (P.cuse) {
         const int csh = P.subpel + 1;
         const int row = P.crow0 + sub * P.clh + (mvy >> csh), col = P.ccol0 + (mvx >> csh);
-        hc = tile_cost(P.csrc, P.cbase + (row * P.crs + col), P.crs, b.satd);
+        hc = tile_cost(P.csrc, P.cbase + (row * P.crs + col), P.crs, P.satd != 0);
     }
     L = group_sum(hl);
     C = 4u * group_sum(hc);
@@ -362,6 +369,30 @@ struct Search {
 __device__ __forceinline__ bool in_bounds(const Search& s, int x, int y) {
     return x >= s.bnd[0] && y >= s.bnd[1] && x < s.bnd[2] && y < s.bnd[3];
 }
+
+// motion_penalty (:1263-1267) of vector (x, y) against the search's predictor.
+__device__ __forceinline__ int64_t motion_of(const Search& s, int x, int y) {
+    const int64_t dx = s.pmx - x, dy = s.pmy - y;
+    return ((int64_t)s.lambda * (dx * dx + dy * dy)) >> 8;
+}
+
+// exact_check's first two tests (:1122-1155): inside the vector bounds, and a motion
+// penalty below the best cost (with pnew >= -256 a candidate's cost is at least its
+// motion penalty, so a point failing this cannot win).
+__device__ __forceinline__ bool survives(const Search& s, int64_t best_cost, int x, int y) {
+    return in_bounds(s, x, y) && motion_of(s, x, y) < best_cost;
+}
+
+// Number of set bits of the wave mask m below this lane.
+__device__ __forceinline__ int lane_rank(uint64_t m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Forward permute: lane `dst` receives v (ds_permute_b32; the whole wave must be active).
+__device__ __forceinline__ int fwd_permute(int dst, int v) { return __builtin_amdgcn_ds_permute(dst << 2, v); }
+
+// Lane 0 of each quad to the whole quad (DPP quad_perm [0, 0, 0, 0]).
+__device__ __forceinline__ uint32_t quad_bcast0(uint32_t v) { return dpp<0x00>(v); }
 
 __device__ __forceinline__ void take_lower(int64_t& c, int& k, int64_t oc, int ok) {
     if (oc < c || (oc == c && ok < k)) { c = oc; k = ok; }
@@ -411,6 +442,7 @@ enum PatKind : int {
     kPatHexStep,    // exact_hex2 iteration: 3 points from odir
     kPatUmh,        // exact_umh (:1036-1119): cross, then 16-point hexagons
     kPatExh,        // exhaustive (2r+1)^2 window, raster order (:771-826)
+    kPatHexExp,     // kPatHexFirst's 6 points, then expanding(1, 1) around the same centre
 };
 
 struct Pattern {
@@ -420,6 +452,7 @@ struct Pattern {
 __device__ __forceinline__ int pat_count(int kind, int r, int step) {
     if (kind == kPatExpand) return 4 * ((2 * r - 1) / step) + 4;
     if (kind == kPatHexFirst) return 6;
+    if (kind == kPatHexExp) return 6 + 8;
     if (kind == kPatHexStep) return 3;
     if (kind == kPatUmh) return 4 * (r / 2) + 16 * max(r / 4, 1);
     return (2 * r + 1) * (2 * r + 1);  // kPatExh
@@ -430,27 +463,41 @@ __device__ __forceinline__ int pat_count(int kind, int r, int step) {
 constexpr uint32_t kHex2X = 0x01343101u, kHex2Y = 0x20024420u;
 constexpr uint64_t kHex4X = 0x6422468888800000ull, kHex4Y = 0x1017876543223456ull;
 
+// Point k of expanding(cx, cy, r, step) (:829-886): the top and bottom rows, the left
+// and right columns, then the four corners.
+__device__ __forceinline__ void pat_expand(int cx, int cy, int r, int step, int k, int& x, int& y) {
+    const unsigned uk = (unsigned)k;
+    const int m = (2 * r - 1) / step;  // values of i in -r+step, -r+2step, ... < r (uniform)
+    if (k < 2 * m) {
+        x = cx - r + step * (1 + (int)(uk >> 1));
+        y = (k & 1) ? cy + r : cy - r;
+    } else if (k < 4 * m) {
+        const unsigned j = uk - 2 * m;
+        y = cy - r + step * (1 + (int)(j >> 1));
+        x = (j & 1) ? cx + r : cx - r;
+    } else {
+        const int c = k - 4 * m;  // (-r,-r), (-r,+r), (+r,-r), (+r,+r)
+        x = (c < 2) ? cx - r : cx + r;
+        y = (c & 1) ? cy + r : cy - r;
+    }
+}
+
+// kHex2[e] around (cx, cy).
+__device__ __forceinline__ void pat_hex(int cx, int cy, unsigned e, int& x, int& y) {
+    x = cx + (int)((kHex2X >> (4 * e)) & 15u) - 2;
+    y = cy + (int)((kHex2Y >> (4 * e)) & 15u) - 2;
+}
+
 __device__ __forceinline__ void pat_gen(const Pattern& p, int k, int& x, int& y) {
     const int cx = p.cx, cy = p.cy, r = p.r;
     const unsigned uk = (unsigned)k;
     if (p.kind == kPatExpand) {
-        const int m = (2 * r - 1) / p.step;  // values of i in -r+step, -r+2step, ... < r (uniform)
-        if (k < 2 * m) {
-            x = cx - r + p.step * (1 + (int)(uk >> 1));
-            y = (k & 1) ? cy + r : cy - r;
-        } else if (k < 4 * m) {
-            const unsigned j = uk - 2 * m;
-            y = cy - r + p.step * (1 + (int)(j >> 1));
-            x = (j & 1) ? cx + r : cx - r;
-        } else {
-            const int c = k - 4 * m;  // (-r,-r), (-r,+r), (+r,-r), (+r,+r)
-            x = (c < 2) ? cx - r : cx + r;
-            y = (c & 1) ? cy + r : cy - r;
-        }
+        pat_expand(cx, cy, r, p.step, k, x, y);
     } else if (p.kind == kPatHexFirst || p.kind == kPatHexStep) {  // kHex2[k + 1], or kHex2[odir + k] (p.step = odir)
-        const unsigned e = p.kind == kPatHexFirst ? uk + 1 : (unsigned)p.step + uk;
-        x = cx + (int)((kHex2X >> (4 * e)) & 15u) - 2;
-        y = cy + (int)((kHex2Y >> (4 * e)) & 15u) - 2;
+        pat_hex(cx, cy, p.kind == kPatHexFirst ? uk + 1 : (unsigned)p.step + uk, x, y);
+    } else if (p.kind == kPatHexExp) {
+        if (k < 6) pat_hex(cx, cy, uk + 1, x, y);
+        else pat_expand(cx, cy, 1, 1, k - 6, x, y);
     } else if (p.kind == kPatUmh) {
         const int nd = r / 2;
         if (k < 2 * nd) {
@@ -662,15 +709,9 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
     }
     const uint64_t smask = __ballot(nseed_lane);
     const int nseeds = __popcll(smask);
-    int seed_x, seed_y;  // seed j in lane 4 + j
-    {
-        const int j = lane - 4;
-        int src = 0, cnt = 0;
-        for (int q = 0; q < 9; q++)
-            if ((smask >> q) & 1) { if (cnt == j) src = q; cnt++; }
-        seed_x = __shfl(sdx, src);
-        seed_y = __shfl(sdy, src);
-    }
+    // seed j (the j-th present neighbour) to lane 4 + j; lanes without one send to lane 63
+    const int sdst = nseed_lane ? 4 + lane_rank(smask) : kWave - 1;
+    const int seed_x = fwd_permute(sdst, sdx), seed_y = fwd_permute(sdst, sdy);
     bool has_prev = false;
     int prx = 0, pry = 0;
     if (ch.prev) {
@@ -739,10 +780,36 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
             // branches below are scalar branches
             P = uniform_load(&P);
             const bool list = P.kind == kPatPred;
-            for (int base = 0; base < P.n; base += kPerPass) {
+            // Long lists (UMH, exhaustive, wide rings) go in windows of 2 * kWave points:
+            // one lane per point drops the points out of bounds or whose motion penalty
+            // already reaches the best cost, and the survivors, in list order, fill the
+            // passes densely.  The pass index below is then a survivor's rank, which
+            // orders them as their list index does.
+            const bool dense = !list && !cur.serial && P.n > kPerPass && P.kind != kPatHexExp;
+            const int wlen = dense ? 2 * kWave : max(P.n, 1);
+            for (int w0 = 0; w0 < P.n; w0 += wlen) {
+                int cnt = P.n, n0 = 0;
+                uint64_t m0 = 0, m1 = 0;  // survivors: points w0 + lane, w0 + kWave + lane
+                if (dense) {
+                    bool a0 = false, a1 = false;
+                    int x, y;
+                    if (w0 + lane < P.n) {
+                        pat_gen(P, w0 + lane, x, y);
+                        a0 = survives(cur, best.cost, x, y);
+                    }
+                    if (w0 + kWave + lane < P.n) {
+                        pat_gen(P, w0 + kWave + lane, x, y);
+                        a1 = survives(cur, best.cost, x, y);
+                    }
+                    m0 = __ballot(a0);
+                    m1 = __ballot(a1);
+                    n0 = __popcll(m0);
+                    cnt = n0 + __popcll(m1);
+                }
+            for (int base = 0; base < cnt; base += kPerPass) {
                 const int k = base + lane / kGroup;
                 int mx = 0, my = 0, tpen = 0;
-                bool valid = k < P.n;
+                bool valid = k < cnt;
                 if (list) {
                     // every lane takes part in the shuffles: a lane past the list end
                     // still holds a candidate another group reads
@@ -752,20 +819,34 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                     tpen = __shfl(lpen, kl);
                     const int tact = __shfl(lact, kl);
                     valid = valid && tact != 0;
+                } else if (dense) {
+                    // the list index of the survivor of rank base + g to lane g * kGroup, then
+                    // to its whole group; the other lanes send to odd lanes, which the
+                    // broadcast overwrites
+                    const int r0 = lane_rank(m0), r1 = n0 + lane_rank(m1);
+                    const bool a0 = (m0 >> lane) & 1, a1 = (m1 >> lane) & 1;
+                    const int d0 = (a0 && r0 >= base && r0 < base + kPerPass) ? (r0 - base) * kGroup : (lane | 1);
+                    const int d1 = (a1 && r1 >= base && r1 < base + kPerPass) ? (r1 - base) * kGroup : (lane | 1);
+                    const int i0 = fwd_permute(d0, w0 + lane), i1 = fwd_permute(d1, w0 + kWave + lane);
+                    const int li = (int)quad_bcast0((uint32_t)(k < n0 ? i0 : i1));
+                    if (valid) {
+                        pat_gen(P, li, mx, my);
+                        // the best may have improved in an earlier pass of the window
+                        valid = survives(cur, best.cost, mx, my);
+                    }
                 } else if (valid) {
                     pat_gen(P, k, mx, my);
                     valid = in_bounds(cur, mx, my);
                     if (valid && !cur.serial) {
                         // cost >= motion (pnew >= -256): a candidate whose motion penalty
                         // reaches the best so far cannot win, as in exact_check (:1122-1155)
-                        const int64_t dx = cur.pmx - mx, dy = cur.pmy - my;
-                        valid = (((int64_t)cur.lambda * (dx * dx + dy * dy)) >> 8) < best.cost;
+                        valid = survives(cur, best.cost, mx, my);
                     }
                 }
                 if (!list && __ballot(valid) == 0) continue;  // no candidate of this pass can win
                 uint32_t L = 0, C = 0;
                 if (valid) {
-                    if (plan_covers(tp, mx, my)) plan_cost(tp, b, mx, my, L, C);
+                    if (plan_covers(tp, mx, my)) plan_cost(tp, mx, my, L, C);
                     else group_cost(b, mx, my, pchroma, part, L, C);
                 }
                 const uint32_t sad = sat_add(L, C);
@@ -774,8 +855,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                     if (list) {
                         cost = (int64_t)sad + (((int64_t)tpen * (int64_t)sad) >> 8);
                     } else {
-                        const int64_t dx = cur.pmx - mx, dy = cur.pmy - my;
-                        motion = ((int64_t)cur.lambda * (dx * dx + dy * dy)) >> 8;
+                        motion = motion_of(cur, mx, my);
                         cost = motion + (int64_t)L + (((int64_t)cur.pnew * (int64_t)L) >> 8) + (int64_t)C +
                                (((int64_t)cur.pnew * (int64_t)C) >> 8);
                     }
@@ -787,12 +867,24 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                     if (lane >= base && lane < base + kPerPass) { lsad = t_sad; lcost = t_cost; }
                 }
                 if (!cur.serial || list) {
-                    // lowest (cost, index) == the reference's running strict-`<` minimum
-                    int64_t mc = valid ? cost : INT64_MAX;
-                    int mk = valid ? k : 0x7FFFFFFF;
+                    // lowest (cost, index) == the reference's running strict-`<` minimum;
+                    // kPatHexExp: the hexagon's 6 points first, and the ring only when none
+                    // of them improves (exact_hex2 then expands around the same centre)
+                    const bool two = P.kind == kPatHexExp;
+                    const bool in1 = valid && (!two || k < 6);
+                    int64_t mc = in1 ? cost : INT64_MAX;
+                    int mk = in1 ? k : 0x7FFFFFFF;
                     group_argmin(mc, mk);
                     mc = rl64(mc, 0);
                     mk = rl(mk, 0);
+                    if (two && !(mk != 0x7FFFFFFF && mc < best.cost)) {
+                        const bool in2 = valid && k >= 6;
+                        mc = in2 ? cost : INT64_MAX;
+                        mk = in2 ? k : 0x7FFFFFFF;
+                        group_argmin(mc, mk);
+                        mc = rl64(mc, 0);
+                        mk = rl(mk, 0);
+                    }
                     if (mk != 0x7FFFFFFF && mc < best.cost) {
                         const int src = (mk - base) * kGroup;
                         best.mvx = rl(mx, src);
@@ -802,8 +894,8 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                         win = mk;
                     }
                 } else {  // pnew < -256: exact_check's skip rules, candidate by candidate
-                    const int cnt = min(kPerPass, P.n - base);
-                    for (int j = 0; j < cnt; j++) {
+                    const int cnt1 = min(kPerPass, P.n - base);
+                    for (int j = 0; j < cnt1; j++) {
                         const int src = j * kGroup;
                         if (!rl((int)valid, src)) continue;
                         if (rl64(motion, src) >= best.cost) continue;
@@ -817,6 +909,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                         }
                     }
                 }
+            }
             }
         }
         // ---------------- act on the result; pick the next list (or finish)
@@ -891,7 +984,10 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                     bmy = best.mvy;
                     if (hex_r > 1) {
                         dir = -2;
-                        P = Pattern{kPatHexFirst, bmx, bmy, 0, 0, 6};
+                        // without a winner among the 6 points exact_hex2 expands around the
+                        // same centre: both lists in one pass (not in the serial replay)
+                        P = cur.serial ? Pattern{kPatHexFirst, bmx, bmy, 0, 0, 6}
+                                       : Pattern{kPatHexExp, bmx, bmy, 1, 1, pat_count(kPatHexExp, 1, 1)};
                         step = kStHexFirstDone;
                     } else {
                         P = Pattern{kPatExpand, bmx, bmy, 1, 1, pat_count(kPatExpand, 1, 1)};
@@ -900,12 +996,16 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                     next = true;
                     break;
                 case kStHexFirstDone:
-                    if (win >= 0) {
+                    if (win >= 0 && win < 6) {
                         dir = win;
                         bmx += kHex2[dir + 1][0];
                         bmy += kHex2[dir + 1][1];
                         hi = 1;
                         step = kStHexLoop;
+                    } else if (P.kind == kPatHexExp) {
+                        // expanding(1, 1) around (bmx, bmy) was evaluated in the same pass,
+                        // and `best` holds its result
+                        step = kStExp1Done;
                     } else {
                         best.mvx = bmx;
                         best.mvy = bmy;
@@ -1019,7 +1119,17 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
     o.dx = (int16_t)best.mvx;
     o.dy = (int16_t)best.mvy;
     o.score = sat_mul(best.sad, smallest ? 3u : 1u);
-    o.luma = luma_dc(b);
+    if (tp.on && tp.dc) {
+        // the plan's source tiles are the block's pixels, all inside the level (the
+        // N x N mean of block_luma_dc)
+        uint32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) sum = __builtin_amdgcn_sad_u8(tp.lsrc[k], 0u, sum);
+        sum = group_sum(sum);
+        o.luma = (uint8_t)(sum >> (tp.n == 8 ? 6 : 4));
+    } else {
+        o.luma = luma_dc(b);
+    }
     o.done = 1;
     return o;
 }

@@ -14,7 +14,9 @@ Super frames and vector payloads stay on the device in per-stream rings.  A ring
 is reused only when no pending output can need its frame, and the session's derived
 state of older frames is released.  For every output frame the loop records the time
 its last input frame arrived and the time its render finished (after the step's
-synchronization); their difference is the frame latency.
+synchronization); their difference is the frame latency.  It includes the time the
+stream waited for admission; the latency counted from the later of that arrival and
+the stream's admission is recorded beside it.
 """
 from __future__ import annotations
 
@@ -71,6 +73,7 @@ class _Stream:
         self.released = 0
         self.next_out = 0
         self.bytes = srv.footprint(spec)
+        self.admitted_at = srv.clock()
 
     def src_of(self, k: int) -> int:
         """Source frame of output k at scene mode 0 (floor of the output time)."""
@@ -117,6 +120,8 @@ class Server:
         self.active: list[_Stream] = []
         self.used = 0
         self.latency: list[float] = []
+        self.latency_admitted: list[float] = []
+        self.t0 = time.perf_counter()
         self.outputs = 0
         self.admitted = 0
         self.dropped = 0
@@ -227,6 +232,7 @@ class Server:
             for k in range(k0, k1):
                 ready = s.arrival_of(min(s.src_of(k) + LOOKAHEAD, s.n - 1))
                 self.latency.append(max(done - ready, 0.0))
+                self.latency_admitted.append(max(done - max(ready, s.admitted_at), 0.0))
                 if self.on_output:
                     self.on_output(s.spec.sid, k, s.out[k - k0])
             count += k1 - k0
@@ -272,8 +278,11 @@ class Server:
                 time.sleep(0.002)
         wall = self.clock()
         lat = np.array(self.latency) * 1000.0 if self.latency else np.zeros(1)
+        lad = np.array(self.latency_admitted) * 1000.0 if self.latency_admitted else np.zeros(1)
         return {"wall_s": round(wall, 3), "outputs": self.outputs, "output_fps": round(self.outputs / wall, 2),
                 "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                "p50_ms_from_admission": round(float(np.percentile(lad, 50)), 2),
+                "p99_ms_from_admission": round(float(np.percentile(lad, 99)), 2),
                 "admitted": self.admitted, "dropped": self.dropped, "finished": self.finished,
                 "still_queued": len(self.queue), "steps": self.steps}
 
