@@ -431,6 +431,22 @@ __device__ __forceinline__ void group_argmin(int64_t& c, int& k) {
     }
 }
 
+// Costs below this pack with the group index (kPerPass = 16 groups) into 32 bits.
+constexpr int64_t kPackedCost = (int64_t)1 << 27;
+static_assert(kPerPass <= 16, "4 bits of group index");
+
+// Minimum of v over the wave (values uniform per group; needs the whole wave active);
+// every lane receives it, as a scalar.
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    v = min(v, dpp<kDppRowRor4>(v));
+    v = min(v, dpp<kDppRowRor8>(v));
+    const auto a = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    v = min((uint32_t)a[0], (uint32_t)a[1]);
+    const auto b2 = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    v = min((uint32_t)b2[0], (uint32_t)b2[1]);
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
 __constant__ int kHex2[8][2] = {{-1, -2}, {-2, 0}, {-1, 2}, {1, 2}, {2, 0}, {1, -2}, {-1, -2}, {-2, 0}};
 __constant__ int kMod6m1[8] = {5, 0, 1, 2, 3, 4, 5, 0};
 
@@ -809,6 +825,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
             for (int base = 0; base < cnt; base += kPerPass) {
                 const int k = base + lane / kGroup;
                 int mx = 0, my = 0, tpen = 0;
+                int64_t motion = INT64_MAX;
                 bool valid = k < cnt;
                 if (list) {
                     // every lane takes part in the shuffles: a lane past the list end
@@ -831,16 +848,18 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                     const int li = (int)quad_bcast0((uint32_t)(k < n0 ? i0 : i1));
                     if (valid) {
                         pat_gen(P, li, mx, my);
-                        // the best may have improved in an earlier pass of the window
-                        valid = survives(cur, best.cost, mx, my);
+                        // (inside the bounds) the best may have improved in an earlier pass
+                        motion = motion_of(cur, mx, my);
+                        valid = motion < best.cost;
                     }
                 } else if (valid) {
                     pat_gen(P, k, mx, my);
                     valid = in_bounds(cur, mx, my);
-                    if (valid && !cur.serial) {
+                    if (valid) {
+                        motion = motion_of(cur, mx, my);
                         // cost >= motion (pnew >= -256): a candidate whose motion penalty
                         // reaches the best so far cannot win, as in exact_check (:1122-1155)
-                        valid = survives(cur, best.cost, mx, my);
+                        if (!cur.serial) valid = motion < best.cost;
                     }
                 }
                 if (!list && __ballot(valid) == 0) continue;  // no candidate of this pass can win
@@ -850,12 +869,11 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                     else group_cost(b, mx, my, pchroma, part, L, C);
                 }
                 const uint32_t sad = sat_add(L, C);
-                int64_t cost = INT64_MAX, motion = INT64_MAX;
+                int64_t cost = INT64_MAX;
                 if (valid) {
                     if (list) {
                         cost = (int64_t)sad + (((int64_t)tpen * (int64_t)sad) >> 8);
                     } else {
-                        motion = motion_of(cur, mx, my);
                         cost = motion + (int64_t)L + (((int64_t)cur.pnew * (int64_t)L) >> 8) + (int64_t)C +
                                (((int64_t)cur.pnew * (int64_t)C) >> 8);
                     }
@@ -872,18 +890,30 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                     // of them improves (exact_hex2 then expands around the same centre)
                     const bool two = P.kind == kPatHexExp;
                     const bool in1 = valid && (!two || k < 6);
-                    int64_t mc = in1 ? cost : INT64_MAX;
-                    int mk = in1 ? k : 0x7FFFFFFF;
-                    group_argmin(mc, mk);
-                    mc = rl64(mc, 0);
-                    mk = rl(mk, 0);
-                    if (two && !(mk != 0x7FFFFFFF && mc < best.cost)) {
-                        const bool in2 = valid && k >= 6;
-                        mc = in2 ? cost : INT64_MAX;
-                        mk = in2 ? k : 0x7FFFFFFF;
+                    int64_t mc;
+                    int mk;
+                    if (best.cost < kPackedCost) {
+                        // every improving cost is below 2^27: (cost, group) packed in 32 bits
+                        const int g = lane / kGroup;
+                        const bool imp = valid && cost < best.cost;
+                        uint32_t key = wave_min_u32((imp && in1) ? ((uint32_t)cost << 4) | (uint32_t)g : ~0u);
+                        if (two && key == ~0u) key = wave_min_u32((imp && k >= 6) ? ((uint32_t)cost << 4) | (uint32_t)g : ~0u);
+                        mc = key == ~0u ? INT64_MAX : (int64_t)(key >> 4);
+                        mk = key == ~0u ? 0x7FFFFFFF : base + (int)(key & 15u);
+                    } else {
+                        mc = in1 ? cost : INT64_MAX;
+                        mk = in1 ? k : 0x7FFFFFFF;
                         group_argmin(mc, mk);
                         mc = rl64(mc, 0);
                         mk = rl(mk, 0);
+                        if (two && !(mk != 0x7FFFFFFF && mc < best.cost)) {
+                            const bool in2 = valid && k >= 6;
+                            mc = in2 ? cost : INT64_MAX;
+                            mk = in2 ? k : 0x7FFFFFFF;
+                            group_argmin(mc, mk);
+                            mc = rl64(mc, 0);
+                            mk = rl(mk, 0);
+                        }
                     }
                     if (mk != 0x7FFFFFFF && mc < best.cost) {
                         const int src = (mk - base) * kGroup;
@@ -1446,6 +1476,9 @@ __global__ void __launch_bounds__(kWave * kTailWaves) tail_kernel(const Chain* _
     const int ci = blockIdx.x;
     const int wave = threadIdx.x / kWave;
     const uint32_t base = chains[ci].base;
+    // the count rebuild runs only on coarse levels with the bad-block pass
+    const LevelGeom* gp = chains[ci].geom;
+    const bool rebuild = gp->level > 0 && gp->bad_sad > 0;
     int s = s0 + 1;
     if (threadIdx.x == 0) {
         cnt[s & 1] = st.tail_count[ci];
@@ -1468,9 +1501,9 @@ __global__ void __launch_bounds__(kWave * kTailWaves) tail_kernel(const Chain* _
             sweep_item(chains, ci, idx, st, qu, s);
         }
         __syncthreads();
-        scan_chain(chains, ci, st, qu, s, scan_sh);
-        __syncthreads();
+        // every wave has read n: the list just run can take sweep s + 2's queue
         if (threadIdx.x == 0) cnt[p] = 0;
+        if (rebuild) scan_chain(chains, ci, st, qu, s, scan_sh);
         __syncthreads();
     }
     if (threadIdx.x == 0 && s - s0 <= kTailCap) st.tail_count[ci] = (uint32_t)(s - s0 - 1);  // sweeps run
