@@ -1169,7 +1169,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
 // pass keeps a running count over the visit order) and as their fallback.
 __global__ void __launch_bounds__(kWave, 3) search_level_kernel(const Chain* __restrict__ chains) {
     const Chain ch = uniform_load(&chains[blockIdx.x]);
-    const LevelGeom g = uniform_load(ch.geom);
+    const LevelGeom& g = ch.geom;
     Blk b;
     init_blk(ch, g, b);
     int bad_count = 0;
@@ -1192,7 +1192,7 @@ __global__ void __launch_bounds__(kWave, 3) search_level_kernel(const Chain* __r
 // sweep flags cleared, sweep counters zeroed.
 __global__ void sweep_init_kernel(const Chain* __restrict__ chains, SweepState st) {
     const Chain ch = chains[blockIdx.y];
-    const LevelGeom g = *ch.geom;
+    const LevelGeom& g = ch.geom;
     const int n = g.gw * g.gh;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int packed = ch.order[i];
@@ -1278,7 +1278,7 @@ __device__ __forceinline__ int scan_walk(const uint32_t* fseq, int lo, int hi, i
 __device__ void scan_chain(const Chain* __restrict__ chains, int ci, const SweepState& st, const Queue& qu, int s,
                            int* sh) {
     const Chain ch = uniform_load(&chains[ci]);
-    const LevelGeom g = uniform_load(ch.geom);
+    const LevelGeom& g = ch.geom;
     if (g.level == 0 || g.bad_sad <= 0) return;
     const uint32_t start = __hip_atomic_load(&st.minpos[ci], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();  // every thread has read minpos before it is cleared below
@@ -1335,7 +1335,7 @@ __global__ void __launch_bounds__(kScanThreads) sweep_scan_kernel(const Chain* _
 __device__ __forceinline__ void sweep_item(const Chain* __restrict__ chains, int ci, int idx, const SweepState& st,
                                            const Queue& qu, int s) {
     const Chain ch = uniform_load(&chains[ci]);
-    const LevelGeom g = uniform_load(ch.geom);
+    const LevelGeom& g = ch.geom;
     Blk b;
     init_blk(ch, g, b);
     bool bad;
@@ -1374,6 +1374,26 @@ __device__ __forceinline__ void sweep_item(const Chain* __restrict__ chains, int
     }
 }
 
+// Position pos of the finest level's visit order, initial_search_order (:302-324): the
+// interior rows 1 .. gh-2 as a meander (even rows left to right), then the top and
+// bottom rows column by column, then the left and right columns row by row.
+__device__ __forceinline__ void level0_order(int pos, int gw, int gh, int& x, int& y) {
+    const int iw = gw - 2, ni = iw * (gh - 2);
+    if (pos < ni) {
+        const int r = pos / iw, t = pos - r * iw;
+        y = 1 + r;
+        x = (y & 1) == 0 ? 1 + t : gw - 2 - t;
+    } else if (pos < ni + 2 * gw) {
+        const int j = pos - ni;
+        x = j >> 1;
+        y = (j & 1) ? gh - 1 : 0;
+    } else {
+        const int j = pos - ni - 2 * gw;
+        y = 1 + (j >> 1);
+        x = (j & 1) ? gw - 1 : 0;
+    }
+}
+
 // Sweep 0: every block of every chain, dispatched in visit order.  Each chain's visit
 // order is cut into `nseg` segments, and the (chain, segment) slots v = ci * nseg + seg
 // are interleaved: item u = k * nslots + v takes position k of slot v.  With nslots a
@@ -1387,13 +1407,20 @@ __global__ void __launch_bounds__(kWave, 2) sweep0_kernel(const Chain* __restric
     for (long long u = blockIdx.x; u < total; u += gridDim.x) {
         const int v = (int)(u % nslots), k = (int)(u / nslots);
         const int ci = v / nseg, seg = v - ci * nseg;
-        const LevelGeom* g = chains[ci].geom;
-        const int n = g->gw * g->gh, len = (n + nseg - 1) / nseg;
+        const LevelGeom* g = &chains[ci].geom;
+        const int gw = g->gw, gh = g->gh;
+        const int n = gw * gh, len = (n + nseg - 1) / nseg;
         const int pos = seg * len + k;
         if (k >= len || pos >= n) continue;
-        const int pk = chains[ci].order[pos];
-        const int bx = pk & 0xFFFF, by = pk >> 16;
-        if (bx >= g->gw || by >= g->gh) continue;
+        int bx, by;
+        if (g->level == 0) {
+            level0_order(pos, gw, gh, bx, by);
+        } else {
+            const int pk = chains[ci].order[pos];
+            bx = pk & 0xFFFF;
+            by = pk >> 16;
+        }
+        if (bx >= gw || by >= gh) continue;
         sweep_item(chains, ci, by * g->gw + bx, st, Queue{nullptr}, 0);
     }
 }
@@ -1408,7 +1435,7 @@ __global__ void validate_kernel(const Chain* __restrict__ chains, SweepState st,
          t += (long long)gridDim.x * blockDim.x) {
         const int ci = (int)(t / max_n), idx = (int)(t - (long long)ci * max_n);
         const Chain& ch = chains[ci];
-        const LevelGeom& g = *ch.geom;
+        const LevelGeom& g = ch.geom;
         const int gw = g.gw, gh = g.gh;
         if (idx >= gw * gh) continue;
         const int bx = idx % gw, by = idx / gw;
@@ -1477,7 +1504,7 @@ __global__ void __launch_bounds__(kWave * kTailWaves) tail_kernel(const Chain* _
     const int wave = threadIdx.x / kWave;
     const uint32_t base = chains[ci].base;
     // the count rebuild runs only on coarse levels with the bad-block pass
-    const LevelGeom* gp = chains[ci].geom;
+    const LevelGeom* gp = &chains[ci].geom;
     const bool rebuild = gp->level > 0 && gp->bad_sad > 0;
     int s = s0 + 1;
     if (threadIdx.x == 0) {

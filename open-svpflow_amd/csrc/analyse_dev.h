@@ -51,7 +51,7 @@ struct Chain {
     const DVec* prev;      // projected opposite-direction field (nullptr at the top level)
     const int32_t* order;  // visit order, packed x | y << 16
     int32_t* opos;         // position of each block in the visit order (raster indexed)
-    const LevelGeom* geom;
+    LevelGeom geom;        // the level's geometry (a copy per chain: one descriptor read gives both)
     const int32_t* global; // (gx, gy) global predictor from the coarser level; nullptr = (0, 0)
     uint32_t base;         // first index of this chain's blocks in the level's flag arrays
 };

@@ -488,7 +488,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
             if (top) ch.order = (const int32_t*)blob_ptr(c.order_top_off);
             else if (l == 0) ch.order = (const int32_t*)blob_ptr(c.order0_off);
             else ch.order = (const int32_t*)(W + cb[ci].order[l]);
-            ch.geom = (const LevelGeom*)blob_ptr(c.geom_off) + l;
+            ch.geom = c.geom[l];
             ch.global = top ? nullptr : (const int32_t*)(W + cb[ci].glob[l]);
             chains[nc++] = ch;
             if (top) {
