@@ -70,8 +70,8 @@ int svpf_profile_enable(svpf_ctx* ctx, int on);
 int svpf_profile_read(svpf_ctx* ctx, const char* kernel, double* total_ms, int32_t* launches);
 /* Analyse sweep statistics of one pyramid level since profiling was enabled (no
  * counterpart in the reference; synchronizes while enabled): out4 = {level runs,
- * global sweeps launched, sweeps of the per-chain tail (largest over the chains),
- * block evaluations of the global sweeps}. */
+ * sweeps launched, block evaluations skipped because nothing the block reads had
+ * changed, block evaluations run (sweep 0 included)}. */
 int svpf_analyse_stats(svpf_ctx* ctx, int32_t level, int64_t* out4);
 
 /* ------------------------------------------------------------------ Super */

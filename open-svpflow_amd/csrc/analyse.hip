@@ -1209,29 +1209,17 @@ __global__ void sweep_init_kernel(const Chain* __restrict__ chains, SweepState s
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st.minpos[blockIdx.y] = 0;  // the first rebuild covers the whole order
-        st.tail_count[blockIdx.y] = 0;
     }
     if (blockIdx.x == 0 && blockIdx.y == 0)
         for (int k = threadIdx.x; k < st.max_sweeps + 2; k += blockDim.x) st.counts[k] = 0;
 }
 
-// Queue of the next sweep.  Global sweeps (counter == nullptr) append to
-// wl[q][counts[s + 1]++]; the per-chain tail (tail_kernel) appends to the chain's own
-// segment wl[q][base + (*counter)++] with the counter in LDS.
-struct Queue {
-    uint32_t* counter;  // LDS counter of the chain's next list, or nullptr
-};
-
-__device__ __forceinline__ void queue_block(const SweepState& st, const Queue& qu, int ci, const Chain& ch, int n, int s) {
+// Block n of chain ci into sweep s + 1's list wl[q][counts[s + 1]++], once (flags[q]).
+__device__ __forceinline__ void queue_block(const SweepState& st, int ci, const Chain& ch, int n, int s) {
     const int q = (s + 1) & 1;
     if (atomicExch(&st.flags[q][ch.base + n], 1u) == 0u) {
-        if (qu.counter) {
-            const uint32_t pos = atomicAdd(qu.counter, 1u);
-            st.wl[q][ch.base + pos] = ((unsigned long long)ci << 32) | (unsigned)n;
-        } else {
-            const uint32_t pos = atomicAdd(&st.counts[s + 1], 1u);
-            st.wl[q][pos] = ((unsigned long long)ci << 32) | (unsigned)n;
-        }
+        const uint32_t pos = atomicAdd(&st.counts[s + 1], 1u);
+        st.wl[q][pos] = ((unsigned long long)ci << 32) | (unsigned)n;
     }
 }
 
@@ -1275,7 +1263,7 @@ __device__ __forceinline__ int scan_walk(const uint32_t* fseq, int lo, int hi, i
 // from the previous rebuild's counts: chunk 0's input is exact, so chunk t is exact
 // after t iterations and a repeat with no change is the sequential result (usually
 // after one or two iterations, since a sweep changes few costs).
-__device__ void scan_chain(const Chain* __restrict__ chains, int ci, const SweepState& st, const Queue& qu, int s,
+__device__ void scan_chain(const Chain* __restrict__ chains, int ci, const SweepState& st, int s,
                            int* sh) {
     const Chain ch = uniform_load(&chains[ci]);
     const LevelGeom& g = ch.geom;
@@ -1318,7 +1306,7 @@ __device__ void scan_chain(const Chain* __restrict__ chains, int ci, const Sweep
         const int idx = (pk >> 16) * g.gw + (pk & 0xFFFF);
         const int bj = (int)fs[k] > sat_add_i32(g.bad_sad, sat_mul_i32(g.bad_sad, c) / 16) ? 1 : 0;
         st.cpre[ch.base + idx] = c;
-        if ((uint32_t)bj != st.badf[ch.base + idx]) queue_block(st, qu, ci, ch, idx, s);
+        if ((uint32_t)bj != st.badf[ch.base + idx]) queue_block(st, ci, ch, idx, s);
         c += bj;
     }
     if (tid == 0) st.minpos[ci] = 0xFFFFFFFFu;
@@ -1327,13 +1315,13 @@ __device__ void scan_chain(const Chain* __restrict__ chains, int ci, const Sweep
 __global__ void __launch_bounds__(kScanThreads) sweep_scan_kernel(const Chain* __restrict__ chains, SweepState st,
                                                                    int s) {
     __shared__ int sh[kScanThreads + kScanThreads / kWave];
-    scan_chain(chains, blockIdx.x, st, Queue{nullptr}, s, sh);
+    scan_chain(chains, blockIdx.x, st, s, sh);
 }
 
 // Evaluates block idx of chain ci in sweep s; when its vector (or score) changes, the
 // result is stored and every 3x3 neighbour visited after it is queued for sweep s + 1.
 __device__ __forceinline__ void sweep_item(const Chain* __restrict__ chains, int ci, int idx, const SweepState& st,
-                                           const Queue& qu, int s) {
+                                           int s) {
     const Chain ch = uniform_load(&chains[ci]);
     const LevelGeom& g = ch.geom;
     Blk b;
@@ -1369,7 +1357,7 @@ __device__ __forceinline__ void sweep_item(const Chain* __restrict__ chains, int
         const int nx = bx - 1 + lane % 3, ny = by - 1 + lane / 3;
         if (nx >= 0 && ny >= 0 && nx < g.gw && ny < g.gh) {
             const int n = ny * g.gw + nx;
-            if (ch.opos[n] > me) queue_block(st, qu, ci, ch, n, s);
+            if (ch.opos[n] > me) queue_block(st, ci, ch, n, s);
         }
     }
 }
@@ -1421,7 +1409,7 @@ __global__ void __launch_bounds__(kWave, 2) sweep0_kernel(const Chain* __restric
             by = pk >> 16;
         }
         if (bx >= gw || by >= gh) continue;
-        sweep_item(chains, ci, by * g->gw + bx, st, Queue{nullptr}, 0);
+        sweep_item(chains, ci, by * g->gw + bx, st, 0);
     }
 }
 
@@ -1452,7 +1440,7 @@ __global__ void validate_kernel(const Chain* __restrict__ chains, SweepState st,
                 stale = *reinterpret_cast<const uint32_t*>(&ch.res[n]) != r.seed[q];
             }
         }
-        if (stale) queue_block(st, Queue{}, ci, ch, idx, 0);
+        if (stale) queue_block(st, ci, ch, idx, 0);
     }
 }
 
@@ -1466,74 +1454,8 @@ __global__ void __launch_bounds__(kWave, 2) sweep_kernel(const Chain* __restrict
         const int ci = __builtin_amdgcn_readfirstlane((int)(item >> 32));
         const int idx = __builtin_amdgcn_readfirstlane((int)(item & 0xFFFFFFFFu));
         if (lane_id() == 0) st.flags[p][chains[ci].base + idx] = 0;
-        sweep_item(chains, ci, idx, st, Queue{nullptr}, s);
+        sweep_item(chains, ci, idx, st, s);
     }
-}
-
-// Hand-over to the per-chain tail before sweep s: its list (global, parity s & 1)
-// moves into per-chain segments wl[q][base + k] of the other parity q = (s + 1) & 1,
-// flagged in flags[q] (clear since sweep s - 1 consumed that list), and
-// tail_count[ci] receives k.  The tail then runs this list as its sweep s + 1.
-__global__ void tail_split_kernel(const Chain* __restrict__ chains, SweepState st, int s) {
-    const uint32_t n = st.counts[s];
-    const int p = s & 1, q2 = (s + 1) & 1;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const unsigned long long item = st.wl[p][i];
-        const int ci = (int)(item >> 32), idx = (int)(item & 0xFFFFFFFFu);
-        const uint32_t base = chains[ci].base;
-        st.flags[p][base + idx] = 0;
-        st.flags[q2][base + idx] = 1;
-        const uint32_t pos = atomicAdd(&st.tail_count[ci], 1u);
-        st.wl[q2][base + pos] = item;
-    }
-}
-
-constexpr int kTailWaves = 8;
-constexpr int kTailCap = 1 << 20;  // sweeps
-
-// The remaining sweeps of one chain in one workgroup: the waves share the chain's
-// list, a workgroup barrier separates the sweeps, and the chain is done when a sweep
-// queues nothing.  Starts from the list that tail_split_kernel left in parity
-// (s0 + 1) & 1, i.e. sweep s0 + 1.
-__global__ void __launch_bounds__(kWave * kTailWaves) tail_kernel(const Chain* __restrict__ chains, SweepState st,
-                                                                  int s0) {
-    static_assert(kWave * kTailWaves == kScanThreads, "the tail's workgroup rebuilds the count");
-    __shared__ uint32_t cnt[2];
-    __shared__ int scan_sh[kScanThreads + kScanThreads / kWave];
-    const int ci = blockIdx.x;
-    const int wave = threadIdx.x / kWave;
-    const uint32_t base = chains[ci].base;
-    // the count rebuild runs only on coarse levels with the bad-block pass
-    const LevelGeom* gp = &chains[ci].geom;
-    const bool rebuild = gp->level > 0 && gp->bad_sad > 0;
-    int s = s0 + 1;
-    if (threadIdx.x == 0) {
-        cnt[s & 1] = st.tail_count[ci];
-        cnt[(s + 1) & 1] = 0;
-    }
-    __syncthreads();
-    for (;; s++) {
-        const int p = s & 1;
-        const uint32_t n = cnt[p];
-        if (n == 0) break;  // uniform: read after the barrier
-        if (s - s0 > kTailCap) {  // never reached (each sweep settles the first dirty position); reported to the host
-            if (threadIdx.x == 0) st.tail_count[ci] = 0xFFFFFFFFu;
-            break;
-        }
-        const Queue qu{&cnt[(s + 1) & 1]};
-        for (uint32_t i = wave; i < n; i += kTailWaves) {
-            const unsigned long long item = st.wl[p][base + i];
-            const int idx = __builtin_amdgcn_readfirstlane((int)(item & 0xFFFFFFFFu));
-            if (lane_id() == 0) st.flags[p][base + idx] = 0;
-            sweep_item(chains, ci, idx, st, qu, s);
-        }
-        __syncthreads();
-        // every wave has read n: the list just run can take sweep s + 2's queue
-        if (threadIdx.x == 0) cnt[p] = 0;
-        if (rebuild) scan_chain(chains, ci, st, qu, s, scan_sh);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0 && s - s0 <= kTailCap) st.tail_count[ci] = (uint32_t)(s - s0 - 1);  // sweeps run
 }
 
 // ------------------------------------------------------------------ level transitions
@@ -1868,11 +1790,6 @@ hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, i
     return hipGetLastError();
 }
 
-hipError_t launch_tail(const Chain* d_chains, int nchains, const SweepState& st, int sweep, hipStream_t s) {
-    hipLaunchKernelGGL(tail_split_kernel, dim3(64), dim3(256), 0, s, d_chains, st, sweep);
-    hipLaunchKernelGGL(tail_kernel, dim3(nchains), dim3(kWave * kTailWaves), 0, s, d_chains, st, sweep);
-    return hipGetLastError();
-}
 
 hipError_t launch_validate(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s) {
     const long long total = (long long)nchains * max_n;

@@ -84,7 +84,6 @@ struct alignas(16) SeedRec {
 // Worklists of the parallel sweeps of one level (all chains together).  Sweep s
 // processes wl[s & 1][0 .. counts[s]) whose members are flagged in flags[s & 1]; a
 // block whose result changes queues its later-visited 3x3 neighbours for sweep s + 1.
-// Once the lists are short, each chain finishes in one workgroup (tail_kernel).
 // Coarse levels with the bad-block pass also keep, per block, the pre-expansion cost
 // (`found`), whether the expansion ran (`badf`) and the running count of bad blocks
 // visited before it (`cpre`, rebuilt after every sweep from the first visit position
@@ -98,7 +97,6 @@ struct SweepState {
     uint32_t* badf;             // per block
     int32_t* cpre;              // per block
     uint32_t* minpos;           // per chain
-    uint32_t* tail_count;       // per chain: list length handed to the per-chain tail
     Memo* memo;                 // per block
     SeedRec* seeds;             // per block
     uint32_t* fseq;             // per block: pre-expansion costs in visit order (count rebuild)

@@ -16,7 +16,6 @@ hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const Sw
 hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s);
 hipError_t launch_sweep_scan(const Chain* d_chains, int nchains, const SweepState& st, int sweep, hipStream_t s);
 hipError_t launch_validate(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s);
-hipError_t launch_tail(const Chain* d_chains, int nchains, const SweepState& st, int sweep, hipStream_t s);
 hipError_t launch_interp(const void* d_tr, int ntr, int max_n, hipStream_t s);
 hipError_t launch_global(const void* d_jobs, int njobs, hipStream_t s);
 hipError_t launch_sort_keys(const void* d_jobs, int njobs, int max_n, hipStream_t s);
@@ -147,18 +146,22 @@ struct ClipPlan {
 };
 
 constexpr int kMaxSweeps = 4096;
-constexpr int kTailPerChain = 4;  // hand over to the per-chain tail below this many queued blocks per chain
 constexpr int kNotConverged = 1;
 
-// Blocks evaluated by the global sweeps 0..s-1, less the evaluations skipped because
-// nothing the block reads had changed (counts[max_sweeps + 1], the tail's skips
-// included).  Profiling only; synchronizes.
-long long count_evaluations(svpf_ctx* ctx, const SweepState& st, int s, size_t total_blocks) {
+// Statistics of one level run for the benchmark (profiling only; synchronizes): sweeps
+// launched, block evaluations of sweeps 0..s-1 that were skipped because nothing the
+// block reads had changed (counts[max_sweeps + 1]), and the evaluations run.
+void record_sweeps(svpf_ctx* ctx, const SweepState& st, int s, size_t total_blocks, int level) {
+    long long* t = ctx->sweep_stats[std::min(level, 15)];
+    t[0] += 1;
+    t[1] += s;
     if (hipMemcpy(ctx->h_count, st.counts, 4 * (size_t)(st.max_sweeps + 2), hipMemcpyDeviceToHost) != hipSuccess)
-        return -1;
+        return;
     long long ev = (long long)total_blocks;
     for (int k = 1; k < s; k++) ev += ctx->h_count[k];
-    return ev - (long long)ctx->h_count[st.max_sweeps + 1];
+    const long long skipped = (long long)ctx->h_count[st.max_sweeps + 1];
+    t[2] += skipped;
+    t[3] += ev - skipped;
 }
 
 // Parallel sweeps of one level until no block changes (each sweep evaluates the blocks
@@ -168,7 +171,7 @@ long long count_evaluations(svpf_ctx* ctx, const SweepState& st, int s, size_t t
 // sweeps launched past convergence find an empty queue and return at once.
 int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total_blocks, const SweepState& st,
                bool scan, int level) {
-    const size_t words = (size_t)std::max(kMaxSweeps + 2, nch);
+    const size_t words = (size_t)kMaxSweeps + 2;
     if (ctx->h_count_cap < words) {
         if (ctx->h_count) (void)hipHostFree(ctx->h_count);
         ctx->h_count = nullptr;
@@ -190,32 +193,8 @@ int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total
         s += chunk;
         SVPF_HIP(hipMemcpyAsync(ctx->h_count, st.counts + s, 4, hipMemcpyDeviceToHost, ctx->stream));
         SVPF_HIP(hipStreamSynchronize(ctx->stream));
-        if (*ctx->h_count != 0 && *ctx->h_count <= (uint32_t)nch * kTailPerChain) {
-            // short lists: each chain finishes in its own workgroup, no launches between sweeps
-            SVPF_HIP(launch_tail(dch, nch, st, s, ctx->stream));
-            SVPF_HIP(hipMemcpyAsync(ctx->h_count, st.tail_count, 4 * (size_t)nch, hipMemcpyDeviceToHost, ctx->stream));
-            SVPF_HIP(hipStreamSynchronize(ctx->stream));
-            uint32_t rounds = 0;
-            for (int c = 0; c < nch; c++) {
-                if (ctx->h_count[c] == 0xFFFFFFFFu) return kNotConverged;
-                rounds = std::max(rounds, ctx->h_count[c]);
-            }
-            if (ctx->prof) {
-                long long* t = ctx->sweep_stats[std::min(level, 15)];
-                t[0] += 1;
-                t[1] += s;
-                t[2] += rounds;
-                t[3] += count_evaluations(ctx, st, s, total_blocks);
-            }
-            return SVPF_OK;
-        }
         if (*ctx->h_count == 0) {
-            if (ctx->prof) {  // statistics for the benchmark
-                long long* t = ctx->sweep_stats[std::min(level, 15)];
-                t[0] += 1;
-                t[1] += s;
-                t[3] += count_evaluations(ctx, st, s, total_blocks);
-            }
+            if (ctx->prof) record_sweeps(ctx, st, s, total_blocks, level);
             return SVPF_OK;
         }
         chunk = std::min(chunk * 2, 64);
@@ -347,7 +326,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
     }
     // sweep state per level: flags x2 (one entry per block of every chain), worklists x2, counters
     size_t level_blocks[16] = {}, flags_off[16][2], wl_off[16][2], counts_off[16], found_off[16], badf_off[16],
-        cpre_off[16], minpos_off[16], tailc_off[16], memo_off[16], fseq_off[16], seed_off[16];
+        cpre_off[16], minpos_off[16], memo_off[16], fseq_off[16], seed_off[16];
     int level_max_n[16] = {};
     for (int l = 0; l < max_levels; l++) {
         size_t tot = 0;
@@ -369,7 +348,6 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
         badf_off[l] = work.take(4 * tot + 4);
         cpre_off[l] = work.take(4 * tot + 4);
         minpos_off[l] = work.take(4 * (size_t)nchains + 4);
-        tailc_off[l] = work.take(4 * (size_t)nchains + 4);
         memo_off[l] = work.take(sizeof(Memo) * (tot + 1));
         fseq_off[l] = work.take(4 * tot + 4);
         seed_off[l] = work.take(sizeof(SeedRec) * (tot + 1));
@@ -632,7 +610,6 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
             st.badf = (uint32_t*)(W + badf_off[l]);
             st.cpre = (int32_t*)(W + cpre_off[l]);
             st.minpos = (uint32_t*)(W + minpos_off[l]);
-            st.tail_count = (uint32_t*)(W + tailc_off[l]);
             st.memo = (Memo*)(W + memo_off[l]);
             st.fseq = (uint32_t*)(W + fseq_off[l]);
             st.seeds = (SeedRec*)(W + seed_off[l]);

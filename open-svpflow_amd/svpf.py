@@ -164,10 +164,10 @@ class Context:
         check(lib().svpf_profile_enable(self.h, 1 if on else 0))
 
     def analyse_stats(self, level: int):
-        """{runs, global sweeps, per-chain tail sweeps, global-sweep block evaluations} of one level."""
+        """{runs, sweeps launched, evaluations skipped (inputs unchanged), evaluations run} of one level."""
         out = (C.c_int64 * 4)()
         check(lib().svpf_analyse_stats(self.h, level, out))
-        return dict(zip(("runs", "sweeps", "tail_sweeps", "evaluations"), list(out)))
+        return dict(zip(("runs", "sweeps", "skipped", "evaluations"), list(out)))
 
     def profile_read(self, kernel: str):
         """(total ms, launches) of one timed kernel since profiling was enabled."""
