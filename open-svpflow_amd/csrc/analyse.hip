@@ -15,6 +15,7 @@
 //   * Level transitions (interpolation + opposite-field projection, global predictor,
 //     score sort, child visit order) run as separate kernels over all chains, so a
 //     batch of pairs never returns to the host between levels.
+#include <cmath>
 #include <hipcub/hipcub.hpp>
 
 #include "analyse_cost.h"
@@ -1763,11 +1764,18 @@ hipError_t launch_sweep_init(const Chain* d_chains, int nchains, int max_n, cons
     return hipGetLastError();
 }
 
-hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s) {
+hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s,
+                         int level) {
     // segments per chain: a multiple of q so that the slot count is a multiple of the 8
-    // XCDs, about 1024 slots (few items of a slot in flight at once; more segments put
-    // more blocks at a segment start, where they read neighbours of the previous
-    // segment before those are searched), and no segment much shorter than 2048 blocks
+    // XCDs, and no segment much shorter than 2048 blocks.  Fewer slots put more items of
+    // a slot in flight at once (read before their predecessor is stored); more slots put
+    // more blocks at a segment start (read before the previous segment reaches them).
+    // The slot count that needed the fewest evaluations at the finest level fell with
+    // the batch: about 2000 slots at 32 chains, 1080 at 120, 720 at 240 (1080p, 8x8),
+    // i.e. at most about 2048 * sqrt(32 / chains) (rounded down to whole segments per
+    // chain: 3 segments at 240 chains beat 4 and 5); coarser levels use about 1024.
+    const int target =
+        level == 0 ? (int)std::min(8192.0, std::max(256.0, 2048.0 * std::sqrt(32.0 / nchains))) : 1024;
     int g8 = 8;
     for (int a = nchains % 8; a != 0;) {
         const int t = g8 % a;
@@ -1775,7 +1783,7 @@ hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const Sw
         a = t;
     }
     const int q = 8 / g8;
-    int nseg = q * ((1024 + nchains * q - 1) / (nchains * q));
+    int nseg = q * std::max(target / (nchains * q), 1);
     const int cap = std::max(q, max_n / 2048 / q * q);
     nseg = std::min(nseg, cap);
     const int nslots = nchains * nseg;

@@ -12,7 +12,8 @@ namespace svpf {
 
 hipError_t launch_search_level(const Chain* d_chains, int nchains, hipStream_t s);
 hipError_t launch_sweep_init(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s);
-hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s);
+hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s,
+                         int level);
 hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s);
 hipError_t launch_sweep_scan(const Chain* d_chains, int nchains, const SweepState& st, int sweep, hipStream_t s);
 hipError_t launch_validate(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s);
@@ -179,7 +180,7 @@ int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total
         SVPF_HIP(hipHostMalloc((void**)&ctx->h_count, 4 * words, hipHostMallocDefault));
         ctx->h_count_cap = words;
     }
-    SVPF_HIP(launch_sweep0(dch, nch, max_n, st, ctx->stream));
+    SVPF_HIP(launch_sweep0(dch, nch, max_n, st, ctx->stream, level));
     if (scan) SVPF_HIP(launch_sweep_scan(dch, nch, st, 0, ctx->stream));
     SVPF_HIP(launch_validate(dch, nch, max_n, st, ctx->stream));
     const int grid = (int)std::min<size_t>(std::max<size_t>(total_blocks, 1), 4096);
