@@ -158,7 +158,8 @@ typedef struct svpf_smooth svpf_smooth;
  * keys and defaults, and on rejection SVPF_ERR_OPT with the reference's error text
  * in `err` ("SVSmoothFps: incorrect 'algo' value", ...).  *out_frames receives
  * the output clip length (frame_math.rs:128-135).  Renders YUV420P8 with the
- * reference's integer CPU renderer semantics (renderer.rs). */
+ * reference's integer CPU renderer semantics (renderer.rs).  With ctx and out both
+ * NULL it only validates the options and sets *out_frames (no device work). */
 int svpf_smooth_create(svpf_ctx* ctx, const char* opt, size_t opt_len, int32_t width, int32_t height, int64_t fps_num,
                        int64_t fps_den, int32_t nframes, int64_t sdata, const int32_t* vdata16, svpf_smooth** out,
                        int32_t* out_frames, char* err, size_t err_len);
@@ -177,6 +178,19 @@ int svpf_smooth_set_sources(svpf_smooth* s, int32_t first, int32_t count, const 
  * `algo`; the call then synchronizes. */
 int svpf_smooth_render(svpf_smooth* s, int32_t k0, int32_t count, const svpf_frame* d_out, int32_t* h_decision);
 int32_t svpf_smooth_output_frames(const svpf_smooth* s);
+/* Output frame rate of the session (FilterState::output_info, core.rs:603-613). */
+int svpf_smooth_output_rate(const svpf_smooth* s, int64_t* fps_num, int64_t* fps_den);
+/* What output frame k reads, for a host that fetches frames on demand (the
+ * getFrame requests of FilterState::request_frame, core.rs:150-201): its source
+ * frame (source_frame, core.rs:2054-2077) and the range first..last of source
+ * frames whose planes, Super frames and vector payloads the render of k can read;
+ * raw_phase (Timing::raw_phase_256) and ratio (frame_num / frame_den) feed the
+ * output's duration and PTS (copy_interpolated_timing, frame.rs:111-138). */
+typedef struct svpf_smooth_frame {
+    int32_t source, first, last;
+    double raw_phase, ratio;
+} svpf_smooth_frame;
+int svpf_smooth_frame_info(const svpf_smooth* s, int32_t k, svpf_smooth_frame* out);
 /* Unbinds source frames first..first+count-1 and frees the device state the session
  * derived from them (decoded vectors, motion planes, masks), for callers that stream
  * a long clip through a window of frames (no counterpart in the reference, whose

@@ -17,6 +17,9 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsvpf.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+# VapourSynth plugin shells over libsvpf (one shared library per reference plugin)
+PLUGINS = [("libsvpflow1_vs.so", "svp1.cpp"), ("libsvpflow2_vs.so", "svp2.cpp")]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
           "-Wno-unused-variable", "-I", os.path.join(HERE, "..", "include")]
@@ -52,6 +55,17 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    for name, src in PLUGINS:
+        out = os.path.join(HERE, name)
+        cmd = [CXX, "-O2", "-fPIC", "-std=c++17", "-Wall", "-fvisibility=hidden", "-I", os.path.join(HERE, "..", "include"),
+               "-shared", "-o", out + ".tmp", os.path.join(HERE, "vsplugin", src), "-L", HERE, "-lsvpf",
+               "-Wl,-rpath,$ORIGIN", "-Wl,--version-script=" + os.path.join(HERE, "vsplugin", "exports.map")]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"plugin build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(out + ".tmp", out)
+        if verbose:
+            print("built", out)
     return LIB
 
 

@@ -737,14 +737,16 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
         prx = pv.dx; pry = pv.dy;
     }
 
-    Search s;
-    s.pnew = g.pnew;
-    s.chroma = !smallest;
-    s.serial = g.pnew < -256;
-    s.bnd[0] = -g.pel * px; s.bnd[1] = -g.pel * py;
-    s.bnd[2] = g.pel * (g.lw - px - bw) + 1; s.bnd[3] = g.pel * (g.lh - py - bh) + 1;
-    s.pmx = pred.dx; s.pmy = pred.dy;
-    s.lambda = 0;
+    // the search context; its predictor is the block's own (pred) except where the
+    // refinement takes the previous-field vector
+    Search cur;
+    cur.pnew = g.pnew;
+    cur.chroma = !smallest;
+    cur.serial = g.pnew < -256;
+    cur.bnd[0] = -g.pel * px; cur.bnd[1] = -g.pel * py;
+    cur.bnd[2] = g.pel * (g.lw - px - bw) + 1; cur.bnd[3] = g.pel * (g.lh - py - bh) + 1;
+    cur.pmx = pred.dx; cur.pmy = pred.dy;
+    cur.lambda = 0;
 
     // candidate list held in lanes (lane k = candidate k) for the list steps
     int lx = 0, ly = 0, lpen = 0, lact = 0;
@@ -763,7 +765,6 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
     };
 
     Best best{0, 0, 0xFFFFFFFFu, INT64_MAX};
-    Search cur = s;
     Pattern P;
     bool pchroma;
     int step;
@@ -959,7 +960,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                     next = true;
                     break;
                 case kStPredDone: {
-                    s.lambda = adapt_lambda(smallest ? 0 : g.lambda_base, g.lsad, pred.score);
+                    cur.lambda = adapt_lambda(smallest ? 0 : g.lambda_base, g.lsad, pred.score);
                     if (g.early) {
                         // level 0 with distance 0: the predictor (or zero) is the answer
                         const int src = field_valid ? 0 : 3;
@@ -971,7 +972,6 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                         step = kStEnd;
                         break;
                     }
-                    cur = s;
                     if (has_prev) {
                         const int64_t fc = rl64(lcost, 0), pc = rl64(lcost, 1);
                         if (pc < (field_valid ? fc : INT64_MAX)) { cur.pmx = prx; cur.pmy = pry; }
@@ -1089,7 +1089,8 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                     const int bad_limit = sat_add_i32(g.bad_sad, sat_mul_i32(g.bad_sad, bad_count) / 16);
                     if (level > 0 && idx > 1 && g.bad_sad > 0 && (int)best.sad > bad_limit) {
                         is_bad = true;
-                        cur = s;  // the block's own predictor
+                        cur.pmx = pred.dx;  // the block's own predictor
+                        cur.pmy = pred.dy;
                         if (g.bad_range < 0) {
                             ring = g.pel;
                             step = kStBadRing;

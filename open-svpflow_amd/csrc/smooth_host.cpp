@@ -263,7 +263,8 @@ int svpf_smooth_create(svpf_ctx* ctx, const char* opt, size_t opt_len, int32_t w
         }
         return fail(SVPF_ERR_OPT, msg);
     };
-    if (!ctx || !out || !vdata16 || width < 8 || height < 8 || (width & 1) || (height & 1) || nframes < 1)
+    // ctx == NULL (with out == NULL): validate only (a plugin's create step)
+    if ((!ctx) != (!out) || !vdata16 || width < 8 || height < 8 || (width & 1) || (height & 1) || nframes < 1)
         return fail(SVPF_ERR_ARG, "svpf_smooth_create: bad argument");
     // create-time checks in the order of filter.rs:60-130
     if (fps_num <= 0 || fps_den <= 0) return reject("SVSmoothFps: unable to determine source frame rate");
@@ -305,6 +306,11 @@ int svpf_smooth_create(svpf_ctx* ctx, const char* opt, size_t opt_len, int32_t w
     s->nframes = nframes;
     s->out_frames = s->timing.frames(nframes);
     s->sdata = sdata;
+    if (!ctx) {
+        if (out_frames) *out_frames = s->out_frames;
+        delete s;
+        return SVPF_OK;
+    }
     SmoothGeom& g = s->geom;
     std::memset(&g, 0, sizeof g);
     g.width = width;
@@ -581,6 +587,31 @@ int svpf_smooth_render(svpf_smooth* s, int32_t k0, int32_t count, const svpf_fra
 }
 
 int32_t svpf_smooth_output_frames(const svpf_smooth* s) { return s ? s->out_frames : 0; }
+
+int svpf_smooth_output_rate(const svpf_smooth* s, int64_t* fps_num, int64_t* fps_den) {
+    if (!s || !fps_num || !fps_den) return fail(SVPF_ERR_ARG, "svpf_smooth_output_rate: bad argument");
+    *fps_num = s->timing.fps_num;  // output_info (core.rs:603-613)
+    *fps_den = s->timing.fps_den;
+    return SVPF_OK;
+}
+
+int svpf_smooth_frame_info(const svpf_smooth* s, int32_t k, svpf_smooth_frame* out) {
+    if (!s || !out || k < 0 || k >= s->out_frames) return fail(SVPF_ERR_ARG, "svpf_smooth_frame_info: bad argument");
+    const Opts& o = s->opt;
+    const Timing& t = s->timing;
+    const int N = s->nframes;
+    // FilterState::source_frame (core.rs:2054-2077): nearest except scene modes 0 and 2
+    const bool nearest = k > 0 && !o.blend && o.scene_mode != 0 && o.scene_mode != 2;
+    const int n = t.source_frame(k, nearest);
+    out->source = std::min(std::max(n, 0), N - 1);
+    // what svpf_smooth_render reads for frame k: sources and Super frames n, n + 1 and
+    // the vectors of n - 1 .. n + 1 (algorithm 23's neighbours)
+    out->first = std::max(out->source - 1, 0);
+    out->last = std::min(out->source + 1, N - 1);
+    out->raw_phase = t.frame_num > 0 ? t.raw_phase_256(k, n) : 0.0;  // core.rs:539-542
+    out->ratio = t.frame_den != 0 ? (double)t.frame_num / (double)t.frame_den : 0.0;
+    return SVPF_OK;
+}
 
 int svpf_smooth_release(svpf_smooth* s, int32_t first, int32_t count) {
     if (!s || first < 0 || count < 0 || first + count > s->nframes)

@@ -1,8 +1,14 @@
-"""The C-ABI library loads and exports every symbol include/svpf.h declares (no GPU calls)."""
+"""The C-ABI library loads and exports every symbol include/svpf.h declares, and each
+VapourSynth plugin library exports exactly what include/svpflow_vs.h declares (no GPU
+calls)."""
 import ctypes
 import os
+import re
+import subprocess
 
 import svpf
+
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def test_library_exports_declared_symbols():
@@ -11,6 +17,17 @@ def test_library_exports_declared_symbols():
     lib = ctypes.CDLL(svpf.LIB_PATH)
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, f"declared but not exported: {missing}"
+
+
+def test_plugin_libraries_export_declared_symbols():
+    text = open(os.path.join(HERE, "..", "include", "svpflow_vs.h")).read()
+    declared = sorted(set(re.findall(r"\b(VapourSynthPluginInit|svpGetVersion)\s*\(", text)))
+    assert declared == ["VapourSynthPluginInit", "svpGetVersion"]
+    for name in ("libsvpflow1_vs.so", "libsvpflow2_vs.so"):
+        path = os.path.join(HERE, "..", "open-svpflow_amd", name)
+        out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+        exported = sorted(line.split()[-1] for line in out.splitlines() if line.split()[1] in "TW")
+        assert exported == declared, f"{name} exports {exported}"
 
 
 def test_option_parsing_matches_oracle(oracle):
