@@ -662,25 +662,44 @@ enum Step : int {
 // predictor -- exactly what the reference's `processed` grid selects in its visit
 // order.  Returns the block's vector (uniform across the wave), *bad = whether the
 // coarse-level bad-block expansion ran and *found_out = the cost before it.
+// A search of the refine pass (exact_recalculate, :1326-1451): the rectangle searched,
+// the clipped interpolated start vector (also the motion predictor) and the pass's
+// options; `factor` is 3 on the border (half-size rectangle, score x 3), else 1.
+struct RefineStart {
+    int px, py, bw, bh;
+    int mvx, mvy;
+    int lambda, pnew;  // (lambda >> 2) / factor (0 on the first row), pnew / factor
+    int type, radius;
+    int factor;
+};
+
+template <bool kRefine>
 __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g, Blk& b, int idx,
                                              int bad_count, bool* bad, uint32_t* found_out, Memo* memo,
-                                             SeedRec* rec, bool* unchanged) {
+                                             SeedRec* rec, bool* unchanged, const RefineStart& rs) {
     const int lane = lane_id();
     const int part = lane & (kGroup - 1);
     const int gx = ch.global ? uniform_int(ch.global[0]) : 0, gy = ch.global ? uniform_int(ch.global[1]) : 0;
     const int gw = g.gw, gh = g.gh, level = g.level;
-    const bool smallest = g.smallest != 0;
+    const bool smallest = !kRefine && g.smallest != 0;
     const int bx = idx % gw, by = idx / gw;
-    const int me = uniform_int(ch.opos[idx]);
-    int px = (level > 0 && bx == gw - 1) ? g.lw - g.bw : bx * g.step_x;
-    int py = (level > 0 && by == gh - 1) ? g.lh - g.bh : by * g.step_y;
-    int bw = g.bw, bh = g.bh;
-    if (smallest) { px += g.bw >> 2; py += g.bh >> 2; bw = g.bw >> 1; bh = g.bh >> 1; }
+    const int me = kRefine ? 0 : uniform_int(ch.opos[idx]);
+    int px, py, bw, bh;
+    if constexpr (kRefine) {
+        px = rs.px; py = rs.py; bw = rs.bw; bh = rs.bh;
+    } else {
+        px = (level > 0 && bx == gw - 1) ? g.lw - g.bw : bx * g.step_x;
+        py = (level > 0 && by == gh - 1) ? g.lh - g.bh : by * g.step_y;
+        bw = g.bw; bh = g.bh;
+        if (smallest) { px += g.bw >> 2; py += g.bh >> 2; bw = g.bw >> 1; bh = g.bh >> 1; }
+    }
     b.px = px; b.py = py; b.bw = bw; b.bh = bh;
-    const bool edge = level > 0 && (bx == gw - 1 || by == gh - 1);
+    const bool edge = !kRefine && level > 0 && (bx == gw - 1 || by == gh - 1);
     DVec pred;
     int epx = 0, epy = 0;
-    if (edge) {
+    if constexpr (kRefine) {
+        pred = DVec{(int16_t)rs.mvx, (int16_t)rs.mvy, 0u, 0u, 0u};
+    } else if (edge) {
         const int si = min(by, gh - 2) * gw + min(bx, gw - 2);
         if (uniform_int(ch.opos[si]) < me) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -693,8 +712,8 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
     } else {
         pred = uniform_load(&ch.in[idx]);
     }
-    int radius = g.radius;
-    if (level > 0 && g.adaptive) {
+    int radius = kRefine ? rs.radius : g.radius;
+    if (!kRefine && level > 0 && g.adaptive) {
         const int ax = px - ((px + bw >= g.lw) ? 1 : 0), ay = py - ((py + bh >= g.lh) ? 1 : 0);
         uint32_t act = plane_activity(b.sp0, b.ss0, g.y_off + ay, ax, bw, bh, b.satd) +
                        plane_activity(b.sp1, b.ss1, g.c_off + py / 2, px / 2, bw / 2, bh / 2, b.satd) +
@@ -705,7 +724,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
     int nseed_lane = 0;
     int sdx = 0, sdy = 0;
     uint32_t seed_word = 0;
-    if (lane < 9) {
+    if (!kRefine && lane < 9) {
         const int nx = bx - 1 + lane % 3, ny = by - 1 + lane / 3;
         if (nx >= 0 && ny >= 0 && nx < gw && ny < gh && ch.opos[ny * gw + nx] < me) {
             // device-scope load: the neighbour may have been stored in this launch by
@@ -740,9 +759,9 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
     // the search context; its predictor is the block's own (pred) except where the
     // refinement takes the previous-field vector
     Search cur;
-    cur.pnew = g.pnew;
+    cur.pnew = kRefine ? rs.pnew : g.pnew;
     cur.chroma = !smallest;
-    cur.serial = g.pnew < -256;
+    cur.serial = cur.pnew < -256;
     cur.bnd[0] = -g.pel * px; cur.bnd[1] = -g.pel * py;
     cur.bnd[2] = g.pel * (g.lw - px - bw) + 1; cur.bnd[3] = g.pel * (g.lh - py - bh) + 1;
     cur.pmx = pred.dx; cur.pmy = pred.dy;
@@ -779,7 +798,8 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
     } else {
         make_plan(b, !smallest, part, tp);
         set_pred_list();
-        P = Pattern{kPatPred, 0, 0, 0, 0, 4 + nseeds};
+        // the refine pass measures only its start vector (lane 0's field entry)
+        P = Pattern{kPatPred, 0, 0, 0, 0, kRefine ? 1 : 4 + nseeds};
         pchroma = !smallest;
         step = kStPredDone;
     }
@@ -960,8 +980,8 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                     next = true;
                     break;
                 case kStPredDone: {
-                    cur.lambda = adapt_lambda(smallest ? 0 : g.lambda_base, g.lsad, pred.score);
-                    if (g.early) {
+                    cur.lambda = kRefine ? rs.lambda : adapt_lambda(smallest ? 0 : g.lambda_base, g.lsad, pred.score);
+                    if (!kRefine && g.early) {
                         // level 0 with distance 0: the predictor (or zero) is the answer
                         const int src = field_valid ? 0 : 3;
                         best.mvx = rl(lx, src);
@@ -988,7 +1008,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                             break;
                         }
                     }
-                    const int stype = (edge && level == 1) ? 0 : g.search_type;
+                    const int stype = kRefine ? rs.type : ((edge && level == 1) ? 0 : g.search_type);
                     exp_next = kStRefineDone;
                     if (stype == 2) {
                         hex_r = radius;
@@ -1150,7 +1170,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
     DVec o;
     o.dx = (int16_t)best.mvx;
     o.dy = (int16_t)best.mvy;
-    o.score = sat_mul(best.sad, smallest ? 3u : 1u);
+    o.score = sat_mul(best.sad, kRefine ? (uint32_t)rs.factor : (smallest ? 3u : 1u));
     if (tp.on && tp.dc) {
         // the plan's source tiles are the block's pixels, all inside the level (the
         // N x N mean of block_luma_dc)
@@ -1183,7 +1203,8 @@ __global__ void __launch_bounds__(kWave, 3) search_level_kernel(const Chain* __r
         const int idx = by * g.gw + bx;
         bool bad;
         uint32_t found;
-        const DVec o = search_block(ch, g, b, idx, bad_count, &bad, &found, nullptr, nullptr, nullptr);
+        const DVec o = search_block<false>(ch, g, b, idx, bad_count, &bad, &found, nullptr, nullptr, nullptr,
+                                           RefineStart{});
         bad_count += bad ? 1 : 0;
         if (threadIdx.x == 0) ch.res[idx] = o;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -1331,8 +1352,8 @@ __device__ __forceinline__ void sweep_item(const Chain* __restrict__ chains, int
     bool bad;
     uint32_t found;
     bool unchanged = false;
-    const DVec o = search_block(ch, g, b, idx, uniform_int(st.cpre[ch.base + idx]), &bad, &found,
-                                st.memo + ch.base + idx, st.seeds + ch.base + idx, &unchanged);
+    const DVec o = search_block<false>(ch, g, b, idx, uniform_int(st.cpre[ch.base + idx]), &bad, &found,
+                                       st.memo + ch.base + idx, st.seeds + ch.base + idx, &unchanged, RefineStart{});
     if (unchanged) {  // nothing it reads changed since its last evaluation
         if (lane_id() == 0) atomicAdd(&st.counts[st.max_sweeps + 1], 1u);  // skip count (statistics)
         return;
@@ -1458,6 +1479,74 @@ __global__ void __launch_bounds__(kWave, 2) sweep_kernel(const Chain* __restrict
         if (lane_id() == 0) st.flags[p][chains[ci].base + idx] = 0;
         sweep_item(chains, ci, idx, st, s);
     }
+}
+
+// The refine pass (exact_recalculate, :1326-1451): one wavefront per output block of
+// every job (blockIdx.y).  Each block's start vector and parent score are the bilinear
+// interpolation of the searched field at its centre (Euclidean division); blocks on
+// the border search a half-size rectangle with a third of the penalties and threshold.
+// Blocks read nothing the pass writes, so the raster order of the reference's loop is
+// not needed.
+__device__ __forceinline__ int64_t div_euclid(int64_t a, int64_t b) {
+    int64_t q = a / b;
+    if (a % b < 0) q += b > 0 ? -1 : 1;
+    return q;
+}
+
+__global__ void __launch_bounds__(kWave, 2) refine_kernel(const RefineJob* __restrict__ jobs) {
+    const Chain ch = uniform_load(&jobs[blockIdx.y].ch);
+    const LevelGeom& g = ch.geom;
+    const int idx = (int)blockIdx.x;
+    if (idx >= g.gw * g.gh) return;
+    const int ogw = uniform_int(jobs[blockIdx.y].old_gw), ogh = uniform_int(jobs[blockIdx.y].old_gh);
+    const int obw = uniform_int(jobs[blockIdx.y].old_bw), obh = uniform_int(jobs[blockIdx.y].old_bh);
+    const int osx = uniform_int(jobs[blockIdx.y].old_sx), osy = uniform_int(jobs[blockIdx.y].old_sy);
+    const int thsad = uniform_int(jobs[blockIdx.y].thsad), lam = uniform_int(jobs[blockIdx.y].lambda);
+    const int pnew = uniform_int(jobs[blockIdx.y].pnew), type = uniform_int(jobs[blockIdx.y].type);
+    const int distance = uniform_int(jobs[blockIdx.y].distance);
+    const int bx = idx % g.gw, by = idx / g.gw;
+    const int bw = g.bw, bh = g.bh;
+    const int cx = bw / 2 + g.step_x * bx, cy = bh / 2 + g.step_y * by;
+    const int ox = (cx - obw / 2) / osx, oy = (cy - obh / 2) / osy;
+    const int64_t dx = max(cx - (obw / 2 + osx * ox), 0), dy = max(cy - (obh / 2 + osy * oy), 0);
+    auto at = [&](int x, int y) { return uniform_load(&ch.in[min(max(y, 0), ogh - 1) * ogw + min(max(x, 0), ogw - 1)]); };
+    const DVec p00 = at(ox, oy), p10 = at(ox + 1, oy), p01 = at(ox, oy + 1), p11 = at(ox + 1, oy + 1);
+    auto interp = [&](int64_t a, int64_t b2, int64_t c, int64_t d) {
+        const int64_t top = a * osx + dx * (b2 - a), bottom = c * osx + dx * (d - c);
+        return div_euclid(top + div_euclid(dy * (bottom - top), osy), osx);
+    };
+    const int mvx = (int)interp(p00.dx, p10.dx, p01.dx, p11.dx);
+    const int mvy = (int)interp(p00.dy, p10.dy, p01.dy, p11.dy);
+    const int64_t parent_sad = interp(p00.score, p10.score, p01.score, p11.score) * (int64_t)(bw * bh) /
+                               (int64_t)(obw * obh);
+    const bool border = bx == 0 || by == 0 || bx == g.gw - 1 || by == g.gh - 1;
+    const int factor = (border && bw > 4) ? 3 : 1;
+    const int ebw = factor == 3 ? bw / 2 : bw, ebh = factor == 3 ? bh / 2 : bh;
+    const int epx = bx * g.step_x + (factor == 3 ? bw / 4 : 0), epy = by * g.step_y + (factor == 3 ? bh / 4 : 0);
+    // mv_bounds (:1620-1636), then exact_clip (:1269-1277) of the start vector
+    const int b0 = -g.pel * epx, b1 = -g.pel * epy;
+    const int b2 = g.pel * (g.lw - epx - ebw) + 1, b3 = g.pel * (g.lh - epy - ebh) + 1;
+    const int prx = min(max(mvx, b0), max(b2 - 1, b0)), pry = min(max(mvy, b1), max(b3 - 1, b1));
+    Blk b;
+    init_blk(ch, g, b);
+    DVec o;
+    if (parent_sad > (int64_t)(thsad / factor)) {
+        const RefineStart rs{epx, epy, ebw, ebh, prx, pry, by == 0 ? 0 : (lam >> 2) / factor, pnew / factor,
+                             type, abs(distance), factor};
+        bool bad;
+        uint32_t found;
+        bool unchanged = false;
+        o = search_block<true>(ch, g, b, idx, 0, &bad, &found, nullptr, nullptr, &unchanged, rs);
+    } else {
+        b.px = epx; b.py = epy; b.bw = ebw; b.bh = ebh;
+        const uint32_t sad = (uint32_t)min(max(parent_sad, (int64_t)0), (int64_t)0xFFFFFFFFll);
+        o.dx = (int16_t)prx;
+        o.dy = (int16_t)pry;
+        o.score = sat_mul(sad, (uint32_t)factor);
+        o.luma = luma_dc(b);
+        o.done = 1;
+    }
+    if (lane_id() == 0) ch.res[idx] = o;
 }
 
 // ------------------------------------------------------------------ level transitions
@@ -1791,6 +1880,11 @@ hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const Sw
     const long long total = (long long)nslots * ((max_n + nseg - 1) / nseg);
     const unsigned grid = (unsigned)std::min<long long>(total, 1 << 22);
     hipLaunchKernelGGL(sweep0_kernel, dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_refine(const void* d_jobs, int njobs, int max_n, hipStream_t s) {
+    hipLaunchKernelGGL(refine_kernel, dim3(max_n, njobs), dim3(kWave), 0, s, (const RefineJob*)d_jobs);
     return hipGetLastError();
 }
 

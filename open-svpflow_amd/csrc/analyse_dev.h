@@ -135,6 +135,16 @@ struct OrderJob {
     int cgw, cgh, gw, gh;
 };
 
+// The refine pass of one chain (exact_recalculate, analyse_search.rs:1326-1451): the
+// chain's planes; `in` = its searched finest-level field (old grid), `res` = the refined
+// field; `geom` describes the output grid at the finest level (output block and step,
+// satd = the refine pass's).
+struct RefineJob {
+    Chain ch;
+    int32_t old_gw, old_gh, old_bw, old_bh, old_sx, old_sy;
+    int32_t thsad, lambda, pnew, type, distance;
+};
+
 struct PackJob {
     const DVec* field;      // final level-0 field
     uint8_t* out;           // payload region start (marker position)

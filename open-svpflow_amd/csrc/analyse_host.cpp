@@ -25,6 +25,7 @@ hipError_t segmented_sort(void* d_temp, size_t* temp_bytes, const uint32_t* keys
                           const int* d_end, hipStream_t s);
 hipError_t launch_child_order(const void* d_jobs, int njobs, hipStream_t s);
 hipError_t launch_pack(const void* d_jobs, int njobs, int max_count, int denom, hipStream_t s);
+hipError_t launch_refine(const void* d_jobs, int njobs, int max_n, hipStream_t s);
 
 namespace {
 
@@ -217,7 +218,6 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
         for (size_t k = 0; k < clips.size(); k++)
             if (clips[k].p == p) c = (int)k;
         if (c < 0) {
-            if (p->has_refine) return fail(SVPF_ERR_ARG, "svpf_analyse: refine[] (exact_recalculate) is not supported");
             if (p->levels < 1 || p->levels > 15 || p->levels > p->super_levels)
                 return fail(SVPF_ERR_ARG, "svpf_analyse: unsupported level count");
             if (p->vectors < 1 || p->vectors > 3) return fail(SVPF_ERR_ARG, "svpf_analyse: bad vectors");
@@ -300,6 +300,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
         seg_off[l] = blob.take(sizeof(int) * 2 * (nchains + 1));
     }
     size_t pack_off = blob.take(sizeof(PackJob) * nchains);
+    size_t refine_off = blob.take(sizeof(RefineJob) * nchains);
     size_t init_off = blob.take(sizeof(Transition) * nchains);
     std::vector<uint8_t> host(blob.off, 0);
 
@@ -307,6 +308,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
     Carve work;
     struct ChainBufs {
         size_t field[16], res[16], opos[16], proj[16], key[16], order[16], glob[16];
+        size_t refined;  // the refine pass's field (output grid), clips with refine[] only
         uint32_t base[16];  // first index in the level's sweep flag arrays
     };
     std::vector<ChainBufs> cb(nchains);
@@ -323,6 +325,12 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
             cb[ci].key[l] = work.take(8 * cnt);
             cb[ci].order[l] = work.take(4 * cnt);
             cb[ci].glob[l] = work.take(8);
+        }
+        if (c.p->has_refine) {
+            int bw, bh, ox, oy, gw, gh;
+            output_block(*c.p, &bw, &bh, &ox, &oy);
+            out_grid(*c.p, bw, bh, ox, oy, &gw, &gh);
+            cb[ci].refined = work.take(sizeof(DVec) * (size_t)gw * gh);
         }
     }
     // sweep state per level: flags x2 (one entry per block of every chain), worklists x2, counters
@@ -545,13 +553,13 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
         output_block(p, &bw, &bh, &ox, &oy);
         out_grid(p, bw, bh, ox, oy, &gw, &gh);
         const int count = gw * gh;
-        if (count != c.gw[0] * c.gh[0]) return fail(SVPF_ERR_ARG, "svpf_analyse: output grid mismatch");
+        if (!p.has_refine && count != c.gw[0] * c.gh[0]) return fail(SVPF_ERR_ARG, "svpf_analyse: output grid mismatch");
         // region order: bwd ("previous") first, then fwd ("current")
         const bool is_bwd = (ci & 1) != 0;
         const size_t region = 4 + (size_t)count * 8;
         const size_t off = 0x40 + ((!is_bwd && (p.vectors & 1)) ? region : 0);
         PackJob j{};
-        j.field = (const DVec*)(W + cb[ci].res[0]);
+        j.field = (const DVec*)(W + (p.has_refine ? cb[ci].refined : cb[ci].res[0]));
         j.out = d_payload[pi] + off;
         j.payload = d_payload[pi];
         j.header = (const int32_t*)blob_ptr(c.header_off);
@@ -559,6 +567,53 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
         pj[npack++] = j;
         max_count = std::max(max_count, count);
         if (((uintptr_t)d_payload[pi]) & 3) return fail(SVPF_ERR_ARG, "svpf_analyse: payload must be 4-byte aligned");
+    }
+    // refine pass jobs (exact_recalculate) of the clips that ask for it
+    RefineJob* rj = (RefineJob*)&host[refine_off];
+    int nrefine = 0, max_refine = 0;
+    for (int ci = 0; ci < nchains; ci++) {
+        if (!chain_alive(ci)) continue;
+        const int pi = ci / 2;
+        const ClipPlan& c = clips[clip_of[pi]];
+        const svpf_analyse_params& p = *c.p;
+        if (!p.has_refine) continue;
+        int bw, bh, ox, oy, gw, gh;
+        output_block(p, &bw, &bh, &ox, &oy);
+        out_grid(p, bw, bh, ox, oy, &gw, &gh);
+        RefineJob j{};
+        j.ch.src = to_splanes((ci & 1) ? d_ref[pi] : d_cur[pi], p);
+        j.ch.ref = to_splanes((ci & 1) ? d_cur[pi] : d_ref[pi], p);
+        j.ch.in = (const DVec*)(W + cb[ci].res[0]);
+        j.ch.res = (DVec*)(W + cb[ci].refined);
+        LevelGeom g = c.geom[0];
+        g.gw = gw;
+        g.gh = gh;
+        g.bw = bw;
+        g.bh = bh;
+        g.step_x = std::max(bw - ox, 1);
+        g.step_y = std::max(bh - oy, 1);
+        g.smallest = 0;
+        g.satd = p.refine_satd;
+        g.search_type = p.refine_search_type;
+        g.radius = std::abs(p.refine_distance);
+        g.early = 0;
+        g.bad_sad = 0;
+        g.bad_range = 0;
+        g.adaptive = 0;
+        j.ch.geom = g;
+        j.old_gw = c.gw[0];
+        j.old_gh = c.gh[0];
+        j.old_bw = p.block_w;
+        j.old_bh = p.block_h;
+        j.old_sx = std::max(p.block_w - p.overlap_x, 1);
+        j.old_sy = std::max(p.block_h - p.overlap_y, 1);
+        j.thsad = p.refine_thsad;
+        j.lambda = p.lambda;
+        j.pnew = p.pnew;
+        j.type = p.refine_search_type;
+        j.distance = p.refine_distance;
+        rj[nrefine++] = j;
+        max_refine = std::max(max_refine, gw * gh);
     }
     // rescale_scores denominator per clip: floor(sqrt(W^2 + H^2)) * output block width
     int denom = -1;
@@ -625,6 +680,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
             prof_end(ctx, ph);
         }
     }
+    if (nrefine) SVPF_HIP(launch_refine(blob_ptr(refine_off), nrefine, max_refine, ctx->stream));
     if (denom > 0) {
         SVPF_HIP(launch_pack(blob_ptr(pack_off), npack, max_count, denom, ctx->stream));
     } else {

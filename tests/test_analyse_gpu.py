@@ -99,6 +99,22 @@ def test_analyse_option_paths(gpu_ctx, oracle, aopt):
     _check(gpu_ctx, oracle, frames, "{pel:2}", aopt)
 
 
+@pytest.mark.parametrize("aopt", [
+    # refine pass (exact_recalculate): the main search's type and distance, then explicit
+    # ones; thsad 0 re-searches every block, a huge thsad none; 16x16 parents give 8x8
+    # output blocks whose border blocks search half-size (factor 3)
+    "{block:{w:8,h:8,overlap:2}, main:{search:{type:3,distance:8,satd:true}}, refine:[{thsad:100}]}",
+    "{block:{w:16,h:16,overlap:2}, main:{search:{type:2,distance:6}}, "
+    "refine:[{thsad:50, search:{type:4,distance:2,satd:true}}]}",
+    "{block:{w:16,h:16,overlap:0}, main:{search:{type:2,distance:6}}, refine:[{thsad:0, search:{type:2,distance:4}}]}",
+    "{block:{w:16,h:16,overlap:2}, main:{search:{type:3,distance:8}}, refine:[{thsad:1000000}]}",
+])
+def test_analyse_refine_pass(gpu_ctx, oracle, aopt):
+    print(f"analyse parity (refine): {aopt}", flush=True)
+    frames = synth.clip(320, 192, 3, seed=78, vmax=8.0, nobj=(2, 3), rot_deg=1.0)
+    _check(gpu_ctx, oracle, frames, "{pel:2}", aopt)
+
+
 def test_analyse_batched_padded(gpu_ctx, oracle):
     print("analyse parity: batch of 4 pairs, padded stride", flush=True)
     frames = synth.clip(352, 200, 5, seed=91, vmax=6.0, nobj=(2, 2), rot_deg=0.0)
