@@ -8,6 +8,18 @@
 // timed CPU baseline; the product library never links or calls it.
 //
 // Build: oracle/Makefile (g++ -O3 -ffp-contract=off) -> oracle/_build/liboracle.so
+//
+// Pinning.  The reference holds no golden vectors for this path and cannot be
+// built here, so each part is pinned by checks of its own in tests/ (closed forms,
+// textbook restatements, brute force, invariants): the option parser and defaults,
+// the Super build, the SAD/SATD costs, the exhaustive search, the render of
+// identical sources.  These parts have no such check and rest on transcription
+// review alone -- parity unpinned:
+//   - the visit orders, seeds, predictor penalties, bad-block count, global
+//     predictor, interpolation/projection, adapt_lambda, rescale_scores and the
+//     refine pass (exact_recalculate) of analyse.cpp;
+//   - the coverage splat and TruncDiv, the scene-class thresholds and the
+//     magnitude power for exponents other than 1 of render.cpp / smooth.cpp.
 #pragma once
 #include <cstdint>
 #include <cstddef>

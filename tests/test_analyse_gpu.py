@@ -115,6 +115,14 @@ def test_analyse_refine_pass(gpu_ctx, oracle, aopt):
     _check(gpu_ctx, oracle, frames, "{pel:2}", aopt)
 
 
+def test_analyse_pel4(gpu_ctx, oracle):
+    # quarter-pel Super frames: level 0 reads the subplane (mvy & 1) * pel + (mvx & 1) at mv >> 1,
+    # as the reference does for every pel >= 2 (analyse_search.rs:1740-1747)
+    print("analyse parity: pel 4 super, 8x8 UMH SATD", flush=True)
+    frames = synth.clip(256, 144, 2, seed=79, vmax=6.0, nobj=(1, 2), rot_deg=0.0)
+    _check(gpu_ctx, oracle, frames, "{pel:4, gpu:0}", "{block:{w:8,h:8,overlap:2}, main:{search:{type:3,distance:8,satd:true}}}")
+
+
 def test_analyse_batched_padded(gpu_ctx, oracle):
     print("analyse parity: batch of 4 pairs, padded stride", flush=True)
     frames = synth.clip(352, 200, 5, seed=91, vmax=6.0, nobj=(2, 2), rot_deg=0.0)
