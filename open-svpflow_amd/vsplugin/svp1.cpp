@@ -135,9 +135,10 @@ struct AnalyseFilter {
     svpf_ctx* ctx = nullptr;
     DevFrame cur, ref;
     uint8_t* d_payload = nullptr;
-    std::vector<uint8_t> payload;
 
-    bool search(const Table& t, const FrameRef* f0, const FrameRef* f1) {
+    // The vector payload of the pair (f0, f1) into `payload` (the caller's: frames of one
+    // instance may be requested from several threads at once).
+    bool search(const Table& t, const FrameRef* f0, const FrameRef* f1, std::vector<uint8_t>& payload) {
         std::lock_guard<std::mutex> lock(mu);
         if (!ctx && svpf_create(0, &ctx) != SVPF_OK) return false;
         if (!d_payload) {
@@ -187,14 +188,15 @@ const FrameRef* analyse_get_frame(int n, int activation, void** instance, void**
     const FrameRef* f0 = get(n, f->super_node, fc);
     const FrameRef* f1 = get(n2, f->super_node, fc);
     FrameRef* out = nullptr;
-    bool ok = f0 && f1 && f->search(t, f0, f1);
+    std::vector<uint8_t> payload;
+    bool ok = f0 && f1 && f->search(t, f0, f1, payload);
     if (ok) {
         out = make(f->vi.format, f->payload_len, 1, f0, core);
         uint8_t* row = out ? t.get_write_ptr()(out, 0) : nullptr;
         const int stride = out ? t.get_stride()(out, 0) : 0;
         ok = row && stride >= f->payload_len;
         if (ok) {  // the payload, then zeros to the end of the row (analyse_filter.rs:312-322)
-            std::memcpy(row, f->payload.data(), (size_t)f->payload_len);
+            std::memcpy(row, payload.data(), (size_t)f->payload_len);
             std::memset(row + f->payload_len, 0, (size_t)(stride - f->payload_len));
         }
     }
