@@ -316,8 +316,8 @@ def main():
                   "output_frames": plain_out, "seconds": round(plain_dt, 3)},
         "batched_over_plain": round(value / plain_fps, 2) if plain_fps > 0 else None,
         "gpu_ms_per_step_events": round(gpu_ms / args.steps, 2),
-        "roofline": {"bound": "hbm", "kernel": "Analyse level search (sweep0/validate/sweep kernels of one pyramid "
-                                                "level; integer-VALU bound)",
+        "roofline": {"bound": "hbm", "kernel": "Analyse level search (sweep0/deferred/validate/sweep kernels of one "
+                                                "pyramid level; integer-VALU bound)",
                      "achieved": round(search_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": search_gbs / HBM_PEAK_GBS, "traffic": None,
                      "launches": search_n, "avg_launch_ms": round(search_ms / max(search_n, 1), 3),

@@ -673,10 +673,20 @@ struct RefineStart {
     int factor;
 };
 
-template <bool kRefine>
+// kFast: the tile-plan instantiation, without the general cost path (group_cost,
+// luma_dc), so that it fits three waves per SIMD.  *defer tells the caller what the
+// general instantiation must redo: kDeferWhole -- an edge block (its predictor is
+// measured at the neighbour's position) or a block without a plan; it returns at once,
+// having stored nothing; kDeferExact -- a candidate fell outside the plan's rectangle
+// and was skipped: the result is provisional (close, so that the blocks visited next
+// seed from it), no memo entry is stored and the seed record is invalidated.
+constexpr int kDeferNone = 0, kDeferWhole = 1, kDeferExact = 2;
+
+template <bool kRefine, bool kFast = false>
 __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g, Blk& b, int idx,
                                              int bad_count, bool* bad, uint32_t* found_out, Memo* memo,
-                                             SeedRec* rec, bool* unchanged, const RefineStart& rs) {
+                                             SeedRec* rec, bool* unchanged, const RefineStart& rs,
+                                             int* defer = nullptr) {
     const int lane = lane_id();
     const int part = lane & (kGroup - 1);
     const int gx = ch.global ? uniform_int(ch.global[0]) : 0, gy = ch.global ? uniform_int(ch.global[1]) : 0;
@@ -695,6 +705,12 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
     }
     b.px = px; b.py = py; b.bw = bw; b.bh = bh;
     const bool edge = !kRefine && level > 0 && (bx == gw - 1 || by == gh - 1);
+    if constexpr (kFast) {
+        if (edge && !smallest) {
+            *defer = kDeferWhole;
+            return DVec{};
+        }
+    }
     DVec pred;
     int epx = 0, epy = 0;
     if constexpr (kRefine) {
@@ -797,6 +813,12 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
         step = kStEdgeDone;
     } else {
         make_plan(b, !smallest, part, tp);
+        if constexpr (kFast) {
+            if (!(tp.on && tp.dc)) {
+                *defer = kDeferWhole;
+                return DVec{};
+            }
+        }
         set_pred_list();
         // the refine pass measures only its start vector (lane 0's field entry)
         P = Pattern{kPatPred, 0, 0, 0, 0, kRefine ? 1 : 4 + nseeds};
@@ -808,7 +830,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
     bool is_bad = false;
     // memo of the refinement (blocks off the edge row and column only)
     if (edge) memo = nullptr;
-    bool memo_hit = false;
+    bool memo_hit = false, provisional = false;
     int key_x = 0, key_y = 0;
     for (;;) {
         // ---------------- the evaluation site: kPerPass candidates per pass, one group each
@@ -886,9 +908,15 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                 }
                 if (!list && __ballot(valid) == 0) continue;  // no candidate of this pass can win
                 uint32_t L = 0, C = 0;
+                bool off_plan = false;
                 if (valid) {
                     if (plan_covers(tp, mx, my)) plan_cost(tp, mx, my, L, C);
-                    else group_cost(b, mx, my, pchroma, part, L, C);
+                    else if constexpr (!kFast) group_cost(b, mx, my, pchroma, part, L, C);
+                    else off_plan = true;
+                }
+                if constexpr (kFast) {
+                    if (off_plan) valid = false;
+                    if (__ballot(off_plan) != 0) provisional = true;
                 }
                 const uint32_t sad = sat_add(L, C);
                 int64_t cost = INT64_MAX;
@@ -1101,7 +1129,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                     step = kStRefineDone;
                     break;
                 case kStRefineDone: {
-                    if (memo && !memo_hit && lane == 0)
+                    if (memo && !memo_hit && !provisional && lane == 0)
                         *memo = Memo{(int16_t)key_x, (int16_t)key_y, (int16_t)best.mvx, (int16_t)best.mvy, best.sad, 1u,
                                      best.cost};
                     // bad-block expansion at coarse levels (:607-672), around the zero vector
@@ -1160,7 +1188,10 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
     }
     *bad = is_bad;
     *found_out = found;
-    if (rec) {
+    if (provisional) {
+        *defer = kDeferExact;
+        if (rec && lane == 0) rec->valid = 0;
+    } else if (rec) {
         if (lane < 9) rec->seed[lane] = seed_word;
         if (lane == 0) {
             rec->bad_count = bad_count;
@@ -1179,7 +1210,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
         for (int k = 0; k < 4; k++) sum = __builtin_amdgcn_sad_u8(tp.lsrc[k], 0u, sum);
         sum = group_sum(sum);
         o.luma = (uint8_t)(sum >> (tp.n == 8 ? 6 : 4));
-    } else {
+    } else if constexpr (!kFast) {  // (kFast: every block it finishes has a plan with dc)
         o.luma = luma_dc(b);
     }
     o.done = 1;
@@ -1234,7 +1265,10 @@ __global__ void sweep_init_kernel(const Chain* __restrict__ chains, SweepState s
         st.minpos[blockIdx.y] = 0;  // the first rebuild covers the whole order
     }
     if (blockIdx.x == 0 && blockIdx.y == 0)
-        for (int k = threadIdx.x; k < st.max_sweeps + 2; k += blockDim.x) st.counts[k] = 0;
+        for (int k = threadIdx.x; k < st.max_sweeps + 2; k += blockDim.x) {
+            st.counts[k] = 0;
+            st.dcount[k] = 0;
+        }
 }
 
 // Block n of chain ci into sweep s + 1's list wl[q][counts[s + 1]++], once (flags[q]).
@@ -1343,6 +1377,10 @@ __global__ void __launch_bounds__(kScanThreads) sweep_scan_kernel(const Chain* _
 
 // Evaluates block idx of chain ci in sweep s; when its vector (or score) changes, the
 // result is stored and every 3x3 neighbour visited after it is queued for sweep s + 1.
+// kFast: the tile-plan evaluation; a block it passes on (whole, or with a provisional
+// result stored as any other) goes to the list of sweep s that deferred_kernel
+// completes before sweep s + 1.
+template <bool kFast>
 __device__ __forceinline__ void sweep_item(const Chain* __restrict__ chains, int ci, int idx, const SweepState& st,
                                            int s) {
     const Chain ch = uniform_load(&chains[ci]);
@@ -1352,8 +1390,14 @@ __device__ __forceinline__ void sweep_item(const Chain* __restrict__ chains, int
     bool bad;
     uint32_t found;
     bool unchanged = false;
-    const DVec o = search_block<false>(ch, g, b, idx, uniform_int(st.cpre[ch.base + idx]), &bad, &found,
-                                       st.memo + ch.base + idx, st.seeds + ch.base + idx, &unchanged, RefineStart{});
+    int defer = kDeferNone;
+    const DVec o = search_block<false, kFast>(ch, g, b, idx, uniform_int(st.cpre[ch.base + idx]), &bad, &found,
+                                              st.memo + ch.base + idx, st.seeds + ch.base + idx, &unchanged,
+                                              RefineStart{}, &defer);
+    if (kFast && defer != kDeferNone) {
+        if (lane_id() == 0) st.dl[atomicAdd(&st.dcount[s], 1u)] = ((unsigned long long)ci << 32) | (unsigned)idx;
+        if (defer == kDeferWhole) return;
+    }
     if (unchanged) {  // nothing it reads changed since its last evaluation
         if (lane_id() == 0) atomicAdd(&st.counts[st.max_sweeps + 1], 1u);  // skip count (statistics)
         return;
@@ -1413,8 +1457,12 @@ __device__ __forceinline__ void level0_order(int pos, int gw, int gh, int& x, in
 // its earlier items stored in place.  So a block dispatched after its earlier-visited
 // neighbours have finished already seeds from their searched vectors, and
 // validate_kernel finds its seed record up to date.
-__global__ void __launch_bounds__(kWave, 2) sweep0_kernel(const Chain* __restrict__ chains, SweepState st,
-                                                          int nslots, int nseg, long long total) {
+// kFast: the tile-plan instantiation (three waves per SIMD), which passes the blocks it
+// cannot finish to deferred_kernel.
+template <bool kFast>
+__global__ void __launch_bounds__(kWave, kFast ? 3 : 2) sweep0_kernel(const Chain* __restrict__ chains,
+                                                                     SweepState st, int nslots, int nseg,
+                                                                     long long total) {
     for (long long u = blockIdx.x; u < total; u += gridDim.x) {
         const int v = (int)(u % nslots), k = (int)(u / nslots);
         const int ci = v / nseg, seg = v - ci * nseg;
@@ -1432,7 +1480,7 @@ __global__ void __launch_bounds__(kWave, 2) sweep0_kernel(const Chain* __restric
             by = pk >> 16;
         }
         if (bx >= gw || by >= gh) continue;
-        sweep_item(chains, ci, by * g->gw + bx, st, 0);
+        sweep_item<kFast>(chains, ci, by * g->gw + bx, st, 0);
     }
 }
 
@@ -1467,7 +1515,9 @@ __global__ void validate_kernel(const Chain* __restrict__ chains, SweepState st,
     }
 }
 
-// Sweep s >= 1 over the blocks queued by sweep s - 1.
+// Sweep s >= 1 over the blocks queued by sweep s - 1.  (Its lists are short after the
+// first few sweeps and each sweep waits on its slowest block, so a second, deferred
+// pass per sweep would cost more than three waves per SIMD gain.)
 __global__ void __launch_bounds__(kWave, 2) sweep_kernel(const Chain* __restrict__ chains, SweepState st, int s) {
     const uint32_t n = (uint32_t)uniform_int((int)__atomic_load_n(&st.counts[s], __ATOMIC_RELAXED));
     const int p = s & 1;
@@ -1477,7 +1527,24 @@ __global__ void __launch_bounds__(kWave, 2) sweep_kernel(const Chain* __restrict
         const int ci = __builtin_amdgcn_readfirstlane((int)(item >> 32));
         const int idx = __builtin_amdgcn_readfirstlane((int)(item & 0xFFFFFFFFu));
         if (lane_id() == 0) st.flags[p][chains[ci].base + idx] = 0;
-        sweep_item(chains, ci, idx, st, s);
+        sweep_item<false>(chains, ci, idx, st, s);
+    }
+}
+
+// The blocks of sweep s that the tile-plan instantiation passed on, evaluated with the
+// general cost path before sweep s + 1 starts.  A block passed on whole stored nothing;
+// one with a provisional result has no valid seed record, so it is searched in full,
+// and a result that differs from the provisional one is stored and propagated as any
+// change is.  So the sweep's invariant -- every block whose inputs changed is
+// evaluated again -- holds as for the others.
+__global__ void __launch_bounds__(kWave, 2) deferred_kernel(const Chain* __restrict__ chains, SweepState st,
+                                                            int s) {
+    const uint32_t n = (uint32_t)uniform_int((int)__atomic_load_n(&st.dcount[s], __ATOMIC_RELAXED));
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const unsigned long long item = st.dl[i];
+        const int ci = __builtin_amdgcn_readfirstlane((int)(item >> 32));
+        const int idx = __builtin_amdgcn_readfirstlane((int)(item & 0xFFFFFFFFu));
+        sweep_item<false>(chains, ci, idx, st, s);
     }
 }
 
@@ -1855,17 +1922,22 @@ hipError_t launch_sweep_init(const Chain* d_chains, int nchains, int max_n, cons
 }
 
 hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s,
-                         int level) {
+                         int level, bool fast) {
     // segments per chain: a multiple of q so that the slot count is a multiple of the 8
-    // XCDs, and no segment much shorter than 2048 blocks.  Fewer slots put more items of
-    // a slot in flight at once (read before their predecessor is stored); more slots put
-    // more blocks at a segment start (read before the previous segment reaches them).
-    // The slot count that needed the fewest evaluations at the finest level fell with
-    // the batch: about 2000 slots at 32 chains, 1080 at 120, 720 at 240 (1080p, 8x8),
-    // i.e. at most about 2048 * sqrt(32 / chains) (rounded down to whole segments per
-    // chain: 3 segments at 240 chains beat 4 and 5); coarser levels use about 1024.
-    const int target =
-        level == 0 ? (int)std::min(8192.0, std::max(256.0, 2048.0 * std::sqrt(32.0 / nchains))) : 1024;
+    // XCDs, and no segment much shorter than seg_min blocks.  Fewer slots put more items
+    // of a slot in flight at once (read before their predecessor is stored); more slots
+    // put more blocks at a segment start (read before the previous segment reaches
+    // them).  The slot count that needed the fewest evaluations at the finest level
+    // (1080p, 8x8) fell with the batch.  Two waves per SIMD (the general kernel): about
+    // 2000 slots at 32 chains, 1080 at 120, 720 at 240, i.e. at most about
+    // 2048 * sqrt(32 / chains), rounded down to whole segments per chain.  Three (the
+    // tile-plan kernel): about 3600 slots at 32 chains and again 720 at 240 (3
+    // segments beat 2 and 4), i.e. about 3584 * (32 / chains)^0.8, with segments of
+    // 1024 blocks or more.  Coarser levels use about 1024.
+    const double r = 32.0 / nchains;
+    const double t0 = fast ? 3584.0 * std::pow(r, 0.8) : 2048.0 * std::sqrt(r);
+    const int target = level == 0 ? (int)std::min(8192.0, std::max(256.0, t0)) : 1024;
+    const int seg_min = fast ? 1024 : 2048;
     int g8 = 8;
     for (int a = nchains % 8; a != 0;) {
         const int t = g8 % a;
@@ -1874,12 +1946,15 @@ hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const Sw
     }
     const int q = 8 / g8;
     int nseg = q * std::max(target / (nchains * q), 1);
-    const int cap = std::max(q, max_n / 2048 / q * q);
+    const int cap = std::max(q, max_n / seg_min / q * q);
     nseg = std::min(nseg, cap);
     const int nslots = nchains * nseg;
     const long long total = (long long)nslots * ((max_n + nseg - 1) / nseg);
     const unsigned grid = (unsigned)std::min<long long>(total, 1 << 22);
-    hipLaunchKernelGGL(sweep0_kernel, dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
+    if (fast)
+        hipLaunchKernelGGL(sweep0_kernel<true>, dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
+    else
+        hipLaunchKernelGGL(sweep0_kernel<false>, dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
     return hipGetLastError();
 }
 
@@ -1890,6 +1965,11 @@ hipError_t launch_refine(const void* d_jobs, int njobs, int max_n, hipStream_t s
 
 hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s) {
     hipLaunchKernelGGL(sweep_kernel, dim3(grid), dim3(kWave), 0, s, d_chains, st, sweep);
+    return hipGetLastError();
+}
+
+hipError_t launch_deferred(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s) {
+    hipLaunchKernelGGL(deferred_kernel, dim3(grid), dim3(kWave), 0, s, d_chains, st, sweep);
     return hipGetLastError();
 }
 

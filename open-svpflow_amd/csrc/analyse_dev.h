@@ -99,6 +99,8 @@ struct SweepState {
     uint32_t* minpos;           // per chain
     Memo* memo;                 // per block
     SeedRec* seeds;             // per block
+    unsigned long long* dl;     // blocks the tile-plan kernels passed on in the current sweep
+    uint32_t* dcount;           // dcount[s] = their number in sweep s
     uint32_t* fseq;             // per block: pre-expansion costs in visit order (count rebuild)
     int max_sweeps;
 };

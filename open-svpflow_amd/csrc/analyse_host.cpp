@@ -13,8 +13,9 @@ namespace svpf {
 hipError_t launch_search_level(const Chain* d_chains, int nchains, hipStream_t s);
 hipError_t launch_sweep_init(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s);
 hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s,
-                         int level);
+                         int level, bool fast);
 hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s);
+hipError_t launch_deferred(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s);
 hipError_t launch_sweep_scan(const Chain* d_chains, int nchains, const SweepState& st, int sweep, hipStream_t s);
 hipError_t launch_validate(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s);
 hipError_t launch_interp(const void* d_tr, int ntr, int max_n, hipStream_t s);
@@ -170,9 +171,11 @@ void record_sweeps(svpf_ctx* ctx, const SweepState& st, int s, size_t total_bloc
 // whose earlier-visited neighbours changed in the previous sweep).  The fixed point is
 // unique -- every block depends only on blocks visited before it -- and equals the
 // sequential result.  The host checks the queue length after each chunk of sweeps;
-// sweeps launched past convergence find an empty queue and return at once.
+// sweeps launched past convergence find an empty queue and return at once.  fast: every
+// chain's searched block is a 4x4 or 8x8 square, and sweep 0 (every block of the level)
+// runs the tile-plan kernel, then the general one over the blocks it passed on.
 int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total_blocks, const SweepState& st,
-               bool scan, int level) {
+               bool scan, int level, bool fast) {
     const size_t words = (size_t)kMaxSweeps + 2;
     if (ctx->h_count_cap < words) {
         if (ctx->h_count) (void)hipHostFree(ctx->h_count);
@@ -181,10 +184,11 @@ int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total
         SVPF_HIP(hipHostMalloc((void**)&ctx->h_count, 4 * words, hipHostMallocDefault));
         ctx->h_count_cap = words;
     }
-    SVPF_HIP(launch_sweep0(dch, nch, max_n, st, ctx->stream, level));
+    const int grid = (int)std::min<size_t>(std::max<size_t>(total_blocks, 1), 4096);
+    SVPF_HIP(launch_sweep0(dch, nch, max_n, st, ctx->stream, level, fast));
+    if (fast) SVPF_HIP(launch_deferred(dch, grid, st, 0, ctx->stream));
     if (scan) SVPF_HIP(launch_sweep_scan(dch, nch, st, 0, ctx->stream));
     SVPF_HIP(launch_validate(dch, nch, max_n, st, ctx->stream));
-    const int grid = (int)std::min<size_t>(std::max<size_t>(total_blocks, 1), 4096);
     int s = 1, chunk = 8;
     for (;;) {
         if (s + chunk > st.max_sweeps) return kNotConverged;
@@ -335,7 +339,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
     }
     // sweep state per level: flags x2 (one entry per block of every chain), worklists x2, counters
     size_t level_blocks[16] = {}, flags_off[16][2], wl_off[16][2], counts_off[16], found_off[16], badf_off[16],
-        cpre_off[16], minpos_off[16], memo_off[16], fseq_off[16], seed_off[16];
+        cpre_off[16], minpos_off[16], memo_off[16], fseq_off[16], seed_off[16], dl_off[16];
     int level_max_n[16] = {};
     for (int l = 0; l < max_levels; l++) {
         size_t tot = 0;
@@ -352,7 +356,8 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
             flags_off[l][q] = work.take(4 * tot + 4);
             wl_off[l][q] = work.take(8 * tot + 8);
         }
-        counts_off[l] = work.take(4 * (size_t)(kMaxSweeps + 2));
+        counts_off[l] = work.take(8 * (size_t)(kMaxSweeps + 2));  // counts, then dcount
+        dl_off[l] = work.take(8 * tot + 8);
         found_off[l] = work.take(4 * tot + 4);
         badf_off[l] = work.take(4 * tot + 4);
         cpre_off[l] = work.take(4 * tot + 4);
@@ -442,6 +447,8 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
     };
     int nch_step[16] = {}, ntr_step[16] = {}, nsort_step[16] = {}, max_tr_n[16] = {}, max_sort_n[16] = {};
     bool seq_step[16] = {}, bad_step[16] = {};  // sequential fallback / bad-block count rebuilds
+    bool plan_step[16];                         // every searched block a 4x4 or 8x8 square
+    for (bool& f : plan_step) f = true;
     Transition* inits = (Transition*)&host[init_off];
     int ninit = 0, max_init_n = 0;
     for (int step = max_levels - 1; step >= 0; step--) {
@@ -538,8 +545,14 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
         nch_step[l] = nc;
         ntr_step[l] = nt;
         nsort_step[l] = ns;
-        for (auto& c : clips)
+        for (auto& c : clips) {
             if (l > 0 && l < c.levels && c.geom[l].bad_sad > 0) bad_step[l] = true;
+            if (l < c.levels) {
+                const LevelGeom& g = c.geom[l];
+                const int sw = g.smallest ? g.bw >> 1 : g.bw, sh = g.smallest ? g.bh >> 1 : g.bh;
+                if (sw != sh || (sw != 4 && sw != 8)) plan_step[l] = false;
+            }
+        }
     }
     if (too_large) return fail(SVPF_ERR_ARG, "svpf_analyse: super-frame plane of 2 GiB or more");
     PackJob* pj = (PackJob*)&host[pack_off];
@@ -662,6 +675,8 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
                 st.flags[q] = (uint32_t*)(W + flags_off[l][q]);
             }
             st.counts = (uint32_t*)(W + counts_off[l]);
+            st.dcount = st.counts + kMaxSweeps + 2;
+            st.dl = (unsigned long long*)(W + dl_off[l]);
             st.found = (uint32_t*)(W + found_off[l]);
             st.badf = (uint32_t*)(W + badf_off[l]);
             st.cpre = (int32_t*)(W + cpre_off[l]);
@@ -672,7 +687,8 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
             st.max_sweeps = kMaxSweeps;
             SVPF_HIP(launch_sweep_init(dch, nch_step[l], level_max_n[l], st, ctx->stream));
             int rc2 = seq_step[l] ? SVPF_OK
-                                  : run_sweeps(ctx, dch, nch_step[l], level_max_n[l], level_blocks[l], st, bad_step[l], l);
+                                  : run_sweeps(ctx, dch, nch_step[l], level_max_n[l], level_blocks[l], st, bad_step[l],
+                                               l, plan_step[l]);
             if (rc2 == kNotConverged || seq_step[l])
                 SVPF_HIP(launch_search_level(dch, nch_step[l], ctx->stream));
             else if (rc2)
