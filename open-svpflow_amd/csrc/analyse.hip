@@ -1458,29 +1458,37 @@ __device__ __forceinline__ void level0_order(int pos, int gw, int gh, int& x, in
 // neighbours have finished already seeds from their searched vectors, and
 // validate_kernel finds its seed record up to date.
 // kFast: the tile-plan instantiation (three waves per SIMD), which passes the blocks it
-// cannot finish to deferred_kernel.
-template <bool kFast>
+// cannot finish to deferred_kernel.  kRun: positions per item, searched in order by one
+// wave.  The coarse levels' order lists each sorted parent's children together, so with
+// runs of 4 a block's earlier siblings are mostly stored before it reads them; the
+// finest level's meander is served best by single positions.
+template <bool kFast, int kRun>
 __global__ void __launch_bounds__(kWave, kFast ? 3 : 2) sweep0_kernel(const Chain* __restrict__ chains,
                                                                      SweepState st, int nslots, int nseg,
                                                                      long long total) {
     for (long long u = blockIdx.x; u < total; u += gridDim.x) {
-        const int v = (int)(u % nslots), k = (int)(u / nslots);
+        const int v = (int)(u % nslots), k0 = (int)(u / nslots);
         const int ci = v / nseg, seg = v - ci * nseg;
         const LevelGeom* g = &chains[ci].geom;
         const int gw = g->gw, gh = g->gh;
         const int n = gw * gh, len = (n + nseg - 1) / nseg;
-        const int pos = seg * len + k;
-        if (k >= len || pos >= n) continue;
-        int bx, by;
-        if (g->level == 0) {
-            level0_order(pos, gw, gh, bx, by);
-        } else {
-            const int pk = chains[ci].order[pos];
-            bx = pk & 0xFFFF;
-            by = pk >> 16;
+        for (int j = 0; j < kRun; j++) {
+            const int k = k0 * kRun + j;
+            const int pos = seg * len + k;
+            if (k >= len || pos >= n) break;
+            int bx, by;
+            if (g->level == 0) {
+                level0_order(pos, gw, gh, bx, by);
+            } else {
+                const int pk = chains[ci].order[pos];
+                bx = pk & 0xFFFF;
+                by = pk >> 16;
+            }
+            if (bx >= gw || by >= gh) continue;
+            sweep_item<kFast>(chains, ci, by * g->gw + bx, st, 0);
+            // the next position's seed reads see this block's result
+            if (kRun > 1) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         }
-        if (bx >= gw || by >= gh) continue;
-        sweep_item<kFast>(chains, ci, by * g->gw + bx, st, 0);
     }
 }
 
@@ -1933,10 +1941,12 @@ hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const Sw
     // 2048 * sqrt(32 / chains), rounded down to whole segments per chain.  Three (the
     // tile-plan kernel): about 3600 slots at 32 chains and again 720 at 240 (3
     // segments beat 2 and 4), i.e. about 3584 * (32 / chains)^0.8, with segments of
-    // 1024 blocks or more.  Coarser levels use about 1024.
+    // 1024 blocks or more.  Coarser levels take items of 4 positions (kRun) and about
+    // 2048 slots.
     const double r = 32.0 / nchains;
     const double t0 = fast ? 3584.0 * std::pow(r, 0.8) : 2048.0 * std::sqrt(r);
-    const int target = level == 0 ? (int)std::min(8192.0, std::max(256.0, t0)) : 1024;
+    const int target = level == 0 ? (int)std::min(8192.0, std::max(256.0, t0)) : 2048;
+    const int run = level == 0 ? 1 : 4;
     const int seg_min = fast ? 1024 : 2048;
     int g8 = 8;
     for (int a = nchains % 8; a != 0;) {
@@ -1949,12 +1959,16 @@ hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const Sw
     const int cap = std::max(q, max_n / seg_min / q * q);
     nseg = std::min(nseg, cap);
     const int nslots = nchains * nseg;
-    const long long total = (long long)nslots * ((max_n + nseg - 1) / nseg);
+    const long long total = (long long)nslots * (((max_n + nseg - 1) / nseg + run - 1) / run);
     const unsigned grid = (unsigned)std::min<long long>(total, 1 << 22);
-    if (fast)
-        hipLaunchKernelGGL(sweep0_kernel<true>, dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
+    if (fast && run == 1)
+        hipLaunchKernelGGL((sweep0_kernel<true, 1>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
+    else if (fast)
+        hipLaunchKernelGGL((sweep0_kernel<true, 4>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
+    else if (run == 1)
+        hipLaunchKernelGGL((sweep0_kernel<false, 1>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
     else
-        hipLaunchKernelGGL(sweep0_kernel<false>, dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
+        hipLaunchKernelGGL((sweep0_kernel<false, 4>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
     return hipGetLastError();
 }
 
