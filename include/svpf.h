@@ -69,7 +69,8 @@ int svpf_timer_stop(svpf_ctx* ctx, float* ms);
 int svpf_profile_enable(svpf_ctx* ctx, int on);
 int svpf_profile_read(svpf_ctx* ctx, const char* kernel, double* total_ms, int32_t* launches);
 /* Analyse sweep statistics of one pyramid level since profiling was enabled (no
- * counterpart in the reference; synchronizes while enabled): out4 = {level runs,
+ * counterpart in the reference; gathered on the device while enabled, without a host
+ * round trip; this call waits for the context's stream): out4 = {level runs,
  * sweeps launched, block evaluations skipped because nothing the block reads had
  * changed, block evaluations run (sweep 0 included)}. */
 int svpf_analyse_stats(svpf_ctx* ctx, int32_t level, int64_t* out4);

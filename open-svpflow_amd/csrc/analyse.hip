@@ -1556,6 +1556,30 @@ __global__ void __launch_bounds__(kWave, 2) deferred_kernel(const Chain* __restr
     }
 }
 
+// Statistics of one converged level run (profiling): out4 += {1 run, `sweeps` launched,
+// evaluations skipped because nothing the block reads had changed (counts[max_sweeps +
+// 1]), evaluations run (every block in sweep 0 plus the lists of sweeps 1 .. sweeps - 1,
+// less the skipped ones)}.  Launches on one stream are ordered, so plain adds suffice.
+__global__ void __launch_bounds__(256) sweep_stats_kernel(SweepState st, int sweeps, long long total_blocks,
+                                                          unsigned long long* __restrict__ out4) {
+    __shared__ unsigned long long part[256];
+    unsigned long long v = 0;
+    for (int k = 1 + (int)threadIdx.x; k < sweeps; k += 256) v += st.counts[k];
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const unsigned long long skipped = st.counts[st.max_sweeps + 1];
+        out4[0] += 1;
+        out4[1] += (unsigned long long)sweeps;
+        out4[2] += skipped;
+        out4[3] += (unsigned long long)total_blocks + part[0] - skipped;
+    }
+}
+
 // The refine pass (exact_recalculate, :1326-1451): one wavefront per output block of
 // every job (blockIdx.y).  Each block's start vector and parent score are the bilinear
 // interpolation of the searched field at its centre (Euclidean division); blocks on
@@ -1979,6 +2003,12 @@ hipError_t launch_refine(const void* d_jobs, int njobs, int max_n, hipStream_t s
 
 hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s) {
     hipLaunchKernelGGL(sweep_kernel, dim3(grid), dim3(kWave), 0, s, d_chains, st, sweep);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_stats(const SweepState& st, int sweeps, long long total_blocks, unsigned long long* out4,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(sweep_stats_kernel, dim3(1), dim3(256), 0, s, st, sweeps, total_blocks, out4);
     return hipGetLastError();
 }
 

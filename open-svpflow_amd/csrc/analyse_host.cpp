@@ -15,6 +15,8 @@ hipError_t launch_sweep_init(const Chain* d_chains, int nchains, int max_n, cons
 hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s,
                          int level, bool fast);
 hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s);
+hipError_t launch_sweep_stats(const SweepState& st, int sweeps, long long total_blocks, unsigned long long* out4,
+                              hipStream_t s);
 hipError_t launch_deferred(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s);
 hipError_t launch_sweep_scan(const Chain* d_chains, int nchains, const SweepState& st, int sweep, hipStream_t s);
 hipError_t launch_validate(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s);
@@ -151,21 +153,6 @@ struct ClipPlan {
 constexpr int kMaxSweeps = 4096;
 constexpr int kNotConverged = 1;
 
-// Statistics of one level run for the benchmark (profiling only; synchronizes): sweeps
-// launched, block evaluations of sweeps 0..s-1 that were skipped because nothing the
-// block reads had changed (counts[max_sweeps + 1]), and the evaluations run.
-void record_sweeps(svpf_ctx* ctx, const SweepState& st, int s, size_t total_blocks, int level) {
-    long long* t = ctx->sweep_stats[std::min(level, 15)];
-    t[0] += 1;
-    t[1] += s;
-    if (hipMemcpy(ctx->h_count, st.counts, 4 * (size_t)(st.max_sweeps + 2), hipMemcpyDeviceToHost) != hipSuccess)
-        return;
-    long long ev = (long long)total_blocks;
-    for (int k = 1; k < s; k++) ev += ctx->h_count[k];
-    const long long skipped = (long long)ctx->h_count[st.max_sweeps + 1];
-    t[2] += skipped;
-    t[3] += ev - skipped;
-}
 
 // Parallel sweeps of one level until no block changes (each sweep evaluates the blocks
 // whose earlier-visited neighbours changed in the previous sweep).  The fixed point is
@@ -200,7 +187,9 @@ int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total
         SVPF_HIP(hipMemcpyAsync(ctx->h_count, st.counts + s, 4, hipMemcpyDeviceToHost, ctx->stream));
         SVPF_HIP(hipStreamSynchronize(ctx->stream));
         if (*ctx->h_count == 0) {
-            if (ctx->prof) record_sweeps(ctx, st, s, total_blocks, level);
+            if (ctx->prof)  // statistics for the benchmark, added on the device
+                SVPF_HIP(launch_sweep_stats(st, s, (long long)total_blocks,
+                                            ctx->d_sweep_stats + 4 * std::min(level, 15), ctx->stream));
             return SVPF_OK;
         }
         chunk = std::min(chunk * 2, 64);

@@ -150,6 +150,8 @@ int svpf_create(int device, svpf_ctx** out) {
     if (e == hipSuccess) e = hipEventCreate(&c->t1);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->desc_done, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventRecord(c->desc_done, c->stream);
+    if (e == hipSuccess) e = hipMalloc((void**)&c->d_sweep_stats, 16 * 4 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemsetAsync(c->d_sweep_stats, 0, 16 * 4 * sizeof(unsigned long long), c->stream);
     if (e != hipSuccess) {
         delete c;
         return fail(SVPF_ERR_HIP, std::string("svpf_create: ") + hipGetErrorString(e));
@@ -167,6 +169,7 @@ void svpf_destroy(svpf_ctx* ctx) {
     prof_clear(ctx);
     for (auto e : ctx->prof_pool) (void)hipEventDestroy(e);
     if (ctx->h_count) (void)hipHostFree(ctx->h_count);
+    if (ctx->d_sweep_stats) (void)hipFree(ctx->d_sweep_stats);
     (void)hipEventDestroy(ctx->desc_done);
     (void)hipEventDestroy(ctx->t0);
     (void)hipEventDestroy(ctx->t1);
@@ -327,13 +330,17 @@ int svpf_profile_enable(svpf_ctx* ctx, int on) {
     SVPF_HIP(hipStreamSynchronize(ctx->stream));
     prof_clear(ctx);
     ctx->prof = on != 0;
-    std::memset(ctx->sweep_stats, 0, sizeof ctx->sweep_stats);
+    SVPF_HIP(hipMemsetAsync(ctx->d_sweep_stats, 0, 16 * 4 * sizeof(unsigned long long), ctx->stream));
+    SVPF_HIP(hipStreamSynchronize(ctx->stream));
     return SVPF_OK;
 }
 
 int svpf_analyse_stats(svpf_ctx* ctx, int32_t level, int64_t* out4) {
     if (!ctx || !out4 || level < 0 || level >= 16) return fail(SVPF_ERR_ARG, "svpf_analyse_stats: bad argument");
-    for (int k = 0; k < 4; k++) out4[k] = ctx->sweep_stats[level][k];
+    unsigned long long t[4];
+    SVPF_HIP(hipMemcpyAsync(t, ctx->d_sweep_stats + 4 * level, sizeof t, hipMemcpyDeviceToHost, ctx->stream));
+    SVPF_HIP(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < 4; k++) out4[k] = (int64_t)t[k];
     return SVPF_OK;
 }
 

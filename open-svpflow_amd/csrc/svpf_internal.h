@@ -26,9 +26,10 @@ struct svpf_ctx {
     size_t sort_tmp_cap = 0;
     uint32_t* h_count = nullptr;  // pinned words for reading back sweep queue lengths
     size_t h_count_cap = 0;
-    // Analyse sweep statistics per level since profiling was last enabled:
-    // {level runs, sweeps launched, sweeps with work, block evaluations}
-    long long sweep_stats[16][4] = {};
+    // Analyse sweep statistics per level since profiling was last enabled (device, 16 x 4,
+    // added by a kernel so that profiling adds no host round trip):
+    // {level runs, sweeps launched, evaluations skipped, evaluations run}
+    unsigned long long* d_sweep_stats = nullptr;
     // Optional kernel timing: HIP event pairs recorded around launches on `stream`.
     struct ProfMark {
         int kernel;
