@@ -334,6 +334,12 @@ __device__ __forceinline__ int interp_pair(int a, int b, int total, int shift, c
     return (w[0] * a + w[1] * b) >> shift;
 }
 
+// v / 2 (truncating, as C's division) when `half`, else v: the chroma planes halve the
+// luma motion.  A runtime divisor of 1 or 2 would compile to a full signed division.
+__device__ __forceinline__ int div12(int v, bool half) {
+    return half ? (v + (int)((uint32_t)v >> 31)) >> 1 : v;
+}
+
 __device__ __forceinline__ int mval(const uint16_t* plane, int idx, int count, int t) {  // motion_value + LUT
     const int li = min(idx < count ? (int)plane[idx] : (int)kNeutral, 2047);
     return trunc_div_256((li - 1024) * t);
@@ -404,7 +410,8 @@ __global__ void __launch_bounds__(256) render_kernel(const FrameJob* __restrict_
     const int mw = g.mw, mh = g.mh;
     auto tile_of = [](int v, int o, int b, int size, int n, int& base) -> int {
         if (v < o) { base = 0; return -1; }
-        int t = (v - o) / b;
+        // (v - o) >= 0: a shift when the block size is a power of two
+        int t = (b & (b - 1)) == 0 ? (v - o) >> __builtin_ctz((unsigned)b) : (v - o) / b;
         const int ext = max((size - o + b - 1) / b - 1, 0);  // first tile reaching the plane end
         t = min(t, min(n - 1, ext));
         base = o + b * t;
@@ -422,10 +429,13 @@ __global__ void __launch_bounds__(256) render_kernel(const FrameJob* __restrict_
     const int t = plan.phase, inv = 256 - plan.phase;
     // motion vector of one motion set at this pixel (motion_samples + interpolate_*)
     auto motion = [&](const uint16_t* mx, const uint16_t* my, int tt, int& dx, int& dy) {
-        const int a0 = mval(mx, i00, mc, tt) / p.xdiv, b0 = mval(my, i00, mc, tt) / p.ydiv;
+        const bool hx = p.xdiv == 2, hy = p.ydiv == 2;
+        const int a0 = div12(mval(mx, i00, mc, tt), hx), b0 = div12(mval(my, i00, mc, tt), hy);
         if (!interp) { dx = a0; dy = b0; return; }
-        const int a1 = mval(mx, i01, mc, tt) / p.xdiv, a2 = mval(mx, i10, mc, tt) / p.xdiv, a3 = mval(mx, i11, mc, tt) / p.xdiv;
-        const int b1 = mval(my, i01, mc, tt) / p.ydiv, b2 = mval(my, i10, mc, tt) / p.ydiv, b3 = mval(my, i11, mc, tt) / p.ydiv;
+        const int a1 = div12(mval(mx, i01, mc, tt), hx), a2 = div12(mval(mx, i10, mc, tt), hx);
+        const int a3 = div12(mval(mx, i11, mc, tt), hx);
+        const int b1 = div12(mval(my, i01, mc, tt), hy), b2 = div12(mval(my, i10, mc, tt), hy);
+        const int b3 = div12(mval(my, i11, mc, tt), hy);
         const int rx0 = interp_pair(a0, a2, p.bh, p.ys, wy), rx1 = interp_pair(a1, a3, p.bh, p.ys, wy);
         const int ry0 = interp_pair(b0, b2, p.bh, p.ys, wy), ry1 = interp_pair(b1, b3, p.bh, p.ys, wy);
         dx = interp_pair(rx0, rx1, p.bw, p.xs, wx);
