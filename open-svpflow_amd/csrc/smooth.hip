@@ -217,7 +217,7 @@ __global__ void plan_kernel(const FrameJob* __restrict__ jobs, FramePlan* __rest
 // ------------------------------------------------------------------ coverage masks
 
 __global__ void cov_clear_kernel(const FrameJob* __restrict__ jobs, const SmoothGeom* __restrict__ gp) {
-    const FrameJob j = jobs[blockIdx.y];
+    const FrameJob& j = jobs[blockIdx.y];
     const int len = (gp->mw + 2) * (gp->mh + 2);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < len; i += gridDim.x * blockDim.x) {
         j.cov_acc[0][i] = 0;
@@ -252,7 +252,7 @@ struct TDiv {
 // mask 1 splats set a at threshold phase.  Integer atomics: the sums do not depend on order.
 __global__ void cov_splat_kernel(const FrameJob* __restrict__ jobs, const FramePlan* __restrict__ plans,
                                  const SmoothGeom* __restrict__ gp) {
-    const FrameJob j = jobs[blockIdx.y];
+    const FrameJob& j = jobs[blockIdx.y];
     const FramePlan p = plans[blockIdx.y];
     const SmoothGeom& g = *gp;
     if (p.decision != 2 || p.algo < 21) return;
@@ -268,7 +268,7 @@ __global__ void cov_splat_kernel(const FrameJob* __restrict__ jobs, const FrameP
         const int x = i % g.gw, y = i / g.gw;
         const int16_t* vec = m == 0 ? j.vec_b : j.vec_a;
         const int threshold = m == 0 ? 256 - p.phase : p.phase;
-        int32_t* acc = j.cov_acc[m];
+        int32_t* acc = m == 0 ? j.cov_acc[0] : j.cov_acc[1];  // (an index by lane would use scratch)
         const int dx = dd.div(threshold * (int)vec[2 * i]), dy = dd.div(threshold * (int)vec[2 * i + 1]);
         const int sx = x * x_step + dx, sy = y_step * y + dy;
         const int left = xd.div(sx + (x_bias & (sx >> 31)));
@@ -298,7 +298,7 @@ __global__ void cov_splat_kernel(const FrameJob* __restrict__ jobs, const FrameP
 // window_3x3 + finish_coverage (renderer.rs:3051-3130) and the max of the magnitude masks.
 __global__ void cov_finish_kernel(const FrameJob* __restrict__ jobs, const FramePlan* __restrict__ plans,
                                  const SmoothGeom* __restrict__ gp) {
-    const FrameJob j = jobs[blockIdx.y];
+    const FrameJob& j = jobs[blockIdx.y];
     const FramePlan p = plans[blockIdx.y];
     const SmoothGeom& g = *gp;
     if (p.decision != 2) return;

@@ -27,8 +27,12 @@ struct L0Args {
 // Full-pel region F(r, c) = src[r][min(c, w-1)] for c < stride (super_build.rs:45-67:
 // copy then pad with the edge byte).  Rows y-2..y+3 come from LDS, others (only the
 // first rows, read where the pel fill runs past the last row) from global memory.
+// LDS bytes by address space: a generic pointer here let the compiler merge the LDS and
+// global reads of F into one load through a selected pointer, held in scratch memory.
+typedef __attribute__((address_space(3))) const uint8_t lds_u8;
+
 struct Rows {
-    const uint8_t* lds;  // 6 rows of `pitch` bytes
+    lds_u8* lds;         // 6 rows of `pitch` bytes
     const uint8_t* sp;   // source plane
     int ss, pitch, y, h, w, ds;
     __device__ __forceinline__ uint8_t F(int r, int c) const {
@@ -56,8 +60,10 @@ __global__ void __launch_bounds__(kThreads) super_level0_kernel(L0Args a) {
     const SuperGeom& g = a.g;
     const int w = g.level_w[plane][0], h = g.level_h[plane][0];
     if (y >= h) return;
-    const Planes src = a.src[frame];
-    const Planes dst = a.dst[frame];
+    // references into the descriptor array: a local copy indexed by `plane` would be
+    // placed in scratch memory
+    const Planes& src = a.src[frame];
+    const Planes& dst = a.dst[frame];
     const int ds = dst.stride[plane];
     const int ss = src.stride[plane];
     const uint8_t* sp = src.p[plane];
@@ -71,7 +77,7 @@ __global__ void __launch_bounds__(kThreads) super_level0_kernel(L0Args a) {
         for (int c = threadIdx.x; c < pitch; c += kThreads) lds[k * pitch + c] = s[min(c, w - 1)];
     }
     __syncthreads();
-    const Rows R{lds, sp, ss, pitch, y, h, w, ds};
+    const Rows R{(lds_u8*)lds, sp, ss, pitch, y, h, w, ds};
 
     const int pel = g.pel;
     const int nsub = g.full ? pel * pel : 1;
@@ -81,7 +87,7 @@ __global__ void __launch_bounds__(kThreads) super_level0_kernel(L0Args a) {
 
     for (int sub = 0; sub < nsub; sub++) {
         uint8_t* row = dp + (size_t)(g.level_row[plane][0] + sub * h + y) * ds;
-        auto value = [&](int x) -> uint8_t {
+        auto value = [&](int x) __attribute__((always_inline)) -> uint8_t {
             if (sub == 0) return R.F(y, x);
             if (sub > 3 || x >= w) return 0;
             if (bilinear) {
@@ -101,20 +107,20 @@ __global__ void __launch_bounds__(kThreads) super_level0_kernel(L0Args a) {
                 return avg8(R.F(y, x), b);
             }
             if (bicubic) {  // super_build.rs:95-177 (`half` per pixel)
-                auto vert = [&](int xx) -> uint8_t {
+                auto vert = [&](int xx) __attribute__((always_inline)) -> uint8_t {
                     if (y < 2 || y + 4 >= h) return (y + 1 == h) ? R.F(y, xx) : avg8(R.F(y, xx), R.F(y + 1, xx));
                     int v = (int)R.F(y - 2, xx) + (int)R.F(y + 3, xx) +
                             5 * (4 * (int)R.F(y, xx) - (int)R.F(y - 1, xx) + 4 * (int)R.F(y + 1, xx) -
                                  (int)R.F(y + 2, xx)) + 16;
                     return (uint8_t)min(255, max(0, v >> 5));
                 };
-                auto horiz = [&](auto get, int xx) -> uint8_t {
+                auto horiz = [&](auto get, int xx) __attribute__((always_inline)) -> uint8_t {
                     if (xx < 2 || xx + 4 >= w) return (xx + 1 == w) ? get(xx) : avg8(get(xx), get(xx + 1));
                     int v = (int)get(xx - 2) + (int)get(xx + 3) +
                             5 * (4 * (int)get(xx) - (int)get(xx - 1) + 4 * (int)get(xx + 1) - (int)get(xx + 2)) + 16;
                     return (uint8_t)min(255, max(0, v >> 5));
                 };
-                if (sub == 1) return horiz([&](int xx) { return R.F(y, xx); }, x);
+                if (sub == 1) return horiz([&](int xx) __attribute__((always_inline)) { return R.F(y, xx); }, x);
                 if (sub == 2) return vert(x);
                 return horiz(vert, x);
             }
@@ -152,7 +158,7 @@ __global__ void __launch_bounds__(kThreads) super_reduce_kernel(ReduceArgs a) {
     const int src_w = g.level_w[plane][l], src_h = g.level_h[plane][l];
     const int dst_w = g.level_w[plane][l + 1], dst_h = g.level_h[plane][l + 1];
     if (dy >= dst_h) return;
-    const Planes P = a.dst[frame];
+    const Planes& P = a.dst[frame];
     const int ds = P.stride[plane];
     uint8_t* base = P.p[plane];
     const uint8_t* srow0 = base + (size_t)g.level_row[plane][l] * ds;
