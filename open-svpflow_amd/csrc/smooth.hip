@@ -340,8 +340,13 @@ __device__ __forceinline__ int div12(int v, bool half) {
     return half ? (v + (int)((uint32_t)v >> 31)) >> 1 : v;
 }
 
+// Reads of the frame and motion planes as global memory (the descriptor's pointers are
+// generic, which compiles to flat loads).
+typedef __attribute__((address_space(1))) const uint8_t gm_u8;
+typedef __attribute__((address_space(1))) const uint16_t gm_u16;
+
 __device__ __forceinline__ int mval(const uint16_t* plane, int idx, int count, int t) {  // motion_value + LUT
-    const int li = min(idx < count ? (int)plane[idx] : (int)kNeutral, 2047);
+    const int li = min(idx < count ? (int)((gm_u16*)plane)[idx] : (int)kNeutral, 2047);
     return trunc_div_256((li - 1024) * t);
 }
 
@@ -350,7 +355,7 @@ __device__ __forceinline__ int alpha_abs(const uint8_t* m, int count, const PP& 
     if (p.ox <= x) { ix0 = (x - p.ox) / p.bw; ix1 = ix0 + 1; }
     if (p.oy <= y) { iy0 = (y - p.oy) / p.bh; iy1 = iy0 + 1; }
     const int fx = p.ox < x ? (x - p.ox) % p.bw : 0, fy = p.oy < y ? (y - p.oy) % p.bh : 0;
-    auto at = [&](int xx, int yy) { const int i = yy * stride + xx; return i < count ? (int)m[i] : 0; };
+    auto at = [&](int xx, int yy) { const int i = yy * stride + xx; return i < count ? (int)((gm_u8*)m)[i] : 0; };
     const int top = ((p.bw - fx) * at(ix0, iy0) + fx * at(ix1, iy0)) / p.bw;
     const int bottom = ((p.bw - fx) * at(ix0, iy1) + fx * at(ix1, iy1)) / p.bw;
     return byte_of(((p.bh - fy) * top + fy * bottom) / p.bh);
@@ -363,10 +368,11 @@ struct Src {
 
 __device__ __forceinline__ int sample(const Src& s, const PP& p, int sx, int sy, int dx, int dy) {
     const int x = clampi(sx + dx, 0, p.max_x), y = clampi(sy + dy, 0, p.max_y);
-    if (s.shift == 0) return s.d[(size_t)y * s.stride + x];
+    gm_u8* d = (gm_u8*)s.d;
+    if (s.shift == 0) return d[(size_t)y * s.stride + x];
     const int mask = (1 << s.shift) - 1;
     const int sub = ((y & mask) << s.shift) | (x & mask);
-    return s.d[(size_t)sub * s.span + (size_t)(y >> s.shift) * s.stride + (x >> s.shift)];
+    return d[(size_t)sub * s.span + (size_t)(y >> s.shift) * s.stride + (x >> s.shift)];
 }
 
 __global__ void __launch_bounds__(256) render_kernel(const FrameJob* __restrict__ jobs, const FramePlan* __restrict__ plans,
@@ -443,7 +449,7 @@ __global__ void __launch_bounds__(256) render_kernel(const FrameJob* __restrict_
     };
     auto alpha = [&](const uint8_t* m) -> int {
         if (zero) return alpha_abs(m, mc, mp, mw, px, py);
-        auto at = [&](int i) { return i < mc ? (int)m[i] : 0; };
+        auto at = [&](int i) { return i < mc ? (int)((gm_u8*)m)[i] : 0; };
         const int r0 = interp_pair(at(i00), at(i10), p.bh, p.ys, wy), r1 = interp_pair(at(i01), at(i11), p.bh, p.ys, wy);
         return byte_of(interp_pair(r0, r1, p.bw, p.xs, wx));
     };
