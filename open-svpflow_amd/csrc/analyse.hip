@@ -1460,8 +1460,8 @@ __device__ __forceinline__ void level0_order(int pos, int gw, int gh, int& x, in
 // kFast: the tile-plan instantiation (three waves per SIMD), which passes the blocks it
 // cannot finish to deferred_kernel.  kRun: positions per item, searched in order by one
 // wave.  The coarse levels' order lists each sorted parent's children together, so with
-// runs of 4 a block's earlier siblings are mostly stored before it reads them; the
-// finest level's meander is served best by single positions.
+// runs of 8 (two parents) a block's earlier siblings are mostly stored before it reads
+// them; the finest level's meander is served best by single positions.
 template <bool kFast, int kRun>
 __global__ void __launch_bounds__(kWave, kFast ? 3 : 2) sweep0_kernel(const Chain* __restrict__ chains,
                                                                      SweepState st, int nslots, int nseg,
@@ -1965,12 +1965,13 @@ hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const Sw
     // 2048 * sqrt(32 / chains), rounded down to whole segments per chain.  Three (the
     // tile-plan kernel): about 3600 slots at 32 chains and again 720 at 240 (3
     // segments beat 2 and 4), i.e. about 3584 * (32 / chains)^0.8, with segments of
-    // 1024 blocks or more.  Coarser levels take items of 4 positions (kRun) and about
-    // 2048 slots.
+    // 1024 blocks or more.  Coarser levels take items of 8 positions (kRun) and about
+    // 4096 slots (runs of 4 with 2048 slots: 3% slower at 120 pairs; runs of 16: no
+    // faster there, and twice the latency of a small batch's sweep 0).
     const double r = 32.0 / nchains;
     const double t0 = fast ? 3584.0 * std::pow(r, 0.8) : 2048.0 * std::sqrt(r);
-    const int target = level == 0 ? (int)std::min(8192.0, std::max(256.0, t0)) : 2048;
-    const int run = level == 0 ? 1 : 4;
+    const int target = level == 0 ? (int)std::min(8192.0, std::max(256.0, t0)) : 4096;
+    const int run = level == 0 ? 1 : 8;
     const int seg_min = fast ? 1024 : 2048;
     int g8 = 8;
     for (int a = nchains % 8; a != 0;) {
@@ -1988,11 +1989,11 @@ hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const Sw
     if (fast && run == 1)
         hipLaunchKernelGGL((sweep0_kernel<true, 1>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
     else if (fast)
-        hipLaunchKernelGGL((sweep0_kernel<true, 4>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
+        hipLaunchKernelGGL((sweep0_kernel<true, 8>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
     else if (run == 1)
         hipLaunchKernelGGL((sweep0_kernel<false, 1>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
     else
-        hipLaunchKernelGGL((sweep0_kernel<false, 4>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
+        hipLaunchKernelGGL((sweep0_kernel<false, 8>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
     return hipGetLastError();
 }
 
