@@ -892,8 +892,12 @@ void exact_search_level(const SuperPlanes& src, const SuperPlanes& ref, const An
             return Best{mx, my, s, (int64_t)s + (((int64_t)penalty * (int64_t)s) >> 8)};
         };
         Best best;
+        BlockTrace tr{};
         if (level == 0 && opts.distance == 0) {
             best = field_valid ? special(pmx, pmy, 0) : special(0, 0, 0);
+            tr.stage_mvx = best.mvx;
+            tr.stage_mvy = best.mvy;
+            tr.stage_sad = (int32_t)best.sad;
         } else {
             Best fieldb{};
             if (field_valid) fieldb = special(pmx, pmy, 0);
@@ -915,16 +919,25 @@ void exact_search_level(const SuperPlanes& src, const SuperPlanes& ref, const An
                 if (c < best.cost) best = Best{seeds[k][0], seeds[k][1], s, c};
             }
             best.cost = (int64_t)best.sad;
+            tr.stage_mvx = best.mvx;
+            tr.stage_mvy = best.mvy;
+            tr.stage_sad = (int32_t)best.sad;
             L.pred_x = pmx;
             L.pred_y = pmy;
             L.refine(stype, radius, &best);
         }
+        tr.mpred_x = pmx;
+        tr.mpred_y = pmy;
         // bad-block expansion (:607-672) uses the block's own predictor
         L.pred_x = predictor.dx;
         L.pred_y = predictor.dy;
         uint32_t found = best.sad;
         int32_t bad_limit = sat_add_i32(opts.coarse_bad_sad, sat_mul_i32(opts.coarse_bad_sad, bad_count) / 16);
+        tr.found = (int32_t)found;
+        tr.bad_before = bad_count;
+        tr.bad_limit = bad_limit;
         if (level > 0 && idx > 1 && opts.coarse_bad_sad > 0 && (int32_t)best.sad > bad_limit) {
+            tr.expanded = 1;
             bad_count++;
             if (opts.coarse_bad_range < 0) {
                 int max_r = -opts.coarse_bad_range * pel;
@@ -944,6 +957,21 @@ void exact_search_level(const SuperPlanes& src, const SuperPlanes& ref, const An
         out.luma = block_luma_dc(src, level, px, py, bw, bh);
         field[idx] = out;
         processed[(size_t)(by * (gw + 1) + bx)] = 1;
+        if (counters && counters->trace[counters->dir]) {
+            tr.level = level;
+            tr.dir = counters->dir;
+            tr.bx = bx; tr.by = by; tr.px = px; tr.py = py; tr.bw = bw; tr.bh = bh;
+            tr.edge = edge; tr.smallest = smallest;
+            tr.pred_dx = predictor.dx; tr.pred_dy = predictor.dy; tr.pred_score = (int32_t)predictor.score;
+            tr.has_prev = has_prev;
+            if (has_prev) { tr.prev_dx = prev[idx].dx; tr.prev_dy = prev[idx].dy; }
+            tr.gx = gx; tr.gy = gy;
+            tr.nseeds = nseeds;
+            for (int k = 0; k < nseeds; k++) { tr.seeds[2 * k] = seeds[k][0]; tr.seeds[2 * k + 1] = seeds[k][1]; }
+            tr.lambda = lambda; tr.radius = radius; tr.stype = stype;
+            tr.out_dx = out.dx; tr.out_dy = out.dy; tr.out_score = (int32_t)out.score; tr.out_luma = out.luma;
+            counters->trace[counters->dir]->push_back(tr);
+        }
     }
 }
 
@@ -1002,12 +1030,19 @@ std::vector<Vec8> exact_recalculate(const SuperPlanes& src, const SuperPlanes& r
             Best best{prx, pry, (uint32_t)std::min<int64_t>(std::max<int64_t>(parent_sad, 0), 0xFFFFFFFFll),
                       std::max<int64_t>(parent_sad, 0)};
             int threshold = thsad / factor;
+            BlockTrace tr{};
+            tr.pred_dx = mvx; tr.pred_dy = mvy;
+            tr.pred_score = (int32_t)std::min<int64_t>(std::max<int64_t>(parent_sad, INT32_MIN), INT32_MAX);
+            tr.stage_mvx = prx; tr.stage_mvy = pry; tr.stage_sad = (int32_t)best.sad;
             if (parent_sad > (int64_t)threshold) {
                 best.sad = L.exact_sad(prx, pry);
                 best.cost = (int64_t)best.sad;
                 L.lambda = by == 0 ? 0 : (opts.lambda >> 2) / factor;
                 L.pred_x = prx;
                 L.pred_y = pry;
+                tr.expanded = 1;   // searched
+                tr.stage_sad = (int32_t)best.sad;
+                tr.lambda = L.lambda;
                 L.refine(opts.refine_search_type, std::abs(opts.refine_distance), &best);
             }
             Vec8 o;
@@ -1016,6 +1051,18 @@ std::vector<Vec8> exact_recalculate(const SuperPlanes& src, const SuperPlanes& r
             o.score = sat_mul_u32(best.sad, (uint32_t)factor);
             o.luma = block_luma_dc(src, 0, epx, epy, ebw, ebh);
             out[(size_t)(by * gw + bx)] = o;
+            if (counters && counters->trace[counters->dir]) {
+                tr.level = -1;
+                tr.dir = counters->dir;
+                tr.bx = bx; tr.by = by; tr.px = epx; tr.py = epy; tr.bw = ebw; tr.bh = ebh;
+                tr.edge = factor;
+                tr.radius = std::abs(opts.refine_distance);
+                tr.stype = opts.refine_search_type;
+                tr.mpred_x = prx; tr.mpred_y = pry;
+                tr.found = (int32_t)best.sad;
+                tr.out_dx = o.dx; tr.out_dy = o.dy; tr.out_score = (int32_t)o.score; tr.out_luma = o.luma;
+                counters->trace[counters->dir]->push_back(tr);
+            }
         }
     }
     return out;
@@ -1144,8 +1191,11 @@ void analyse_pair(const SuperPlanes& cur, const SuperPlanes& ref, const AnalyseO
     *has_bwd = do_bwd;
     *has_fwd = do_fwd;
     if (!(do_bwd && do_fwd)) {
+        if (counters) counters->dir = 0;
         if (do_fwd) *fwd_out = search_hierarchy(cur, ref, opts, counters);
+        if (counters) counters->dir = 1;
         if (do_bwd) *bwd_out = search_hierarchy(ref, cur, opts, counters);
+        if (counters) counters->dir = 0;
         return;
     }
     // exact_search_hierarchy_bidir: each direction's level search takes the other
@@ -1153,6 +1203,12 @@ void analyse_pair(const SuperPlanes& cur, const SuperPlanes& ref, const AnalyseO
     Dir fw, bw;
     int fgw = 0, fgh = 0;
     SearchCounters cb;
+    if (counters) {
+        counters->dir = 0;
+        cb.trace[0] = counters->trace[0];
+        cb.trace[1] = counters->trace[1];
+        cb.dir = 1;
+    }
     for (int level = std::max(opts.levels, 1) - 1; level >= 0; level--) {
         int gw, gh;
         level_grid(opts, level, &gw, &gh);
@@ -1189,7 +1245,7 @@ void analyse_pair(const SuperPlanes& cur, const SuperPlanes& ref, const AnalyseO
         fgh = gh;
     }
     auto f = exact_recalculate(cur, ref, opts, std::move(fw.field), fgw, fgh, counters);
-    auto b = exact_recalculate(ref, cur, opts, std::move(bw.field), fgw, fgh, counters);
+    auto b = exact_recalculate(ref, cur, opts, std::move(bw.field), fgw, fgh, counters ? &cb : nullptr);
     int obw, obh, oox, ooy;
     opts.output_block(&obw, &obh, &oox, &ooy);
     rescale_scores(f, opts.width, opts.height, obw);

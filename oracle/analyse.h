@@ -50,11 +50,33 @@ struct SuperPlanes {  // analyse_search.rs:33-66
     void level_size(int level, size_t* w, size_t* h) const;
 };
 
+// One record per block visit of a level search (level >= 0) or of the refine pass
+// (level == -1): the inputs the reference's rules give the block and what it stored.
+// The tests recompute each field from those rules (analyse_search.rs:480-768,
+// :1326-1451) and compare.  All fields are 32-bit; scores are u32 bit patterns.
+struct BlockTrace {
+    int32_t level, dir, bx, by, px, py, bw, bh, edge, smallest;   // dir 0 = cur->ref, 1 = ref->cur
+    int32_t pred_dx, pred_dy, pred_score;                         // predictor as read (score: u32)
+    int32_t has_prev, prev_dx, prev_dy;                           // projected opposite candidate
+    int32_t gx, gy;                                               // global predictor
+    int32_t nseeds, seeds[16];                                    // seeds[2k], seeds[2k+1]
+    int32_t lambda, radius, stype;
+    int32_t stage_mvx, stage_mvy, stage_sad;                      // predictor-stage winner
+    int32_t mpred_x, mpred_y;                                     // motion-penalty centre of the refinement
+    int32_t found, bad_before, bad_limit, expanded;               // bad-block expansion (coarse levels)
+    int32_t out_dx, out_dy, out_score, out_luma;                  // stored vector
+    int32_t pad[13];
+};
+static_assert(sizeof(BlockTrace) == 64 * 4, "BlockTrace is 64 words");
+
 // Instrumentation: candidate cost evaluations per level (index 0..15), both directions,
 // and the pixels each touched (luma + chroma), for the algorithmic-work figure.
+// `trace[dir]`, when set, receives a BlockTrace per block visit of that direction.
 struct SearchCounters {
     uint64_t evals[16] = {};
     uint64_t pixels[16] = {};
+    std::vector<BlockTrace>* trace[2] = {nullptr, nullptr};
+    int dir = 0;
 };
 
 // analyse_pair (analyse_search.rs:68-92).  Fields are gw*gh raster vectors.

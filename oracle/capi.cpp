@@ -160,6 +160,44 @@ int oracle_analyse(const char* opt, size_t len, int64_t sdata, int width, int he
     return 0;
 }
 
+// get_frame_analyse with a per-block trace of both directions (BlockTrace, 64 words
+// each): trace0 receives direction 0 (cur -> ref), trace1 direction 1.  Runs the two
+// directions one after the other.  Returns -1 when a trace buffer is too small (the
+// counts needed are still written).
+int oracle_analyse_trace(const char* opt, size_t len, int64_t sdata, int width, int height, const uint8_t* cy,
+                         int cys, const uint8_t* cu, int cus, const uint8_t* cv, int cvs, const uint8_t* ry, int rys,
+                         const uint8_t* ru, int rus, const uint8_t* rv, int rvs, uint8_t* out, size_t out_cap,
+                         size_t* out_len, BlockTrace* trace0, size_t cap0, size_t* n0, BlockTrace* trace1,
+                         size_t cap1, size_t* n1) {
+    Value v;
+    bool have = parse_opt(opt, len, &v);
+    AnalyseOpts o;
+    if (!analyse_opts_from(have ? &v : nullptr, sdata, nullptr, &o).empty()) return -3;
+    if (width > 0) o.width = width;
+    if (height > 0) o.height = height;
+    size_t yh = (size_t)super_plane_height(o.height, o.pel, o.super_levels, true), uvh = yh / 2;
+    SuperPlanes c{cy, (size_t)cys * yh, (size_t)cys, cu, (size_t)cus * uvh, (size_t)cus, cv, (size_t)cvs * uvh,
+                  (size_t)cvs, (size_t)o.width, (size_t)o.height, o.pel, o.super_levels, true};
+    SuperPlanes r{ry, (size_t)rys * yh, (size_t)rys, ru, (size_t)rus * uvh, (size_t)rus, rv, (size_t)rvs * uvh,
+                  (size_t)rvs, (size_t)o.width, (size_t)o.height, o.pel, o.super_levels, true};
+    bool hb, hf;
+    std::vector<Vec8> bwd, fwd;
+    std::vector<BlockTrace> t0, t1;
+    SearchCounters sc;
+    sc.trace[0] = &t0;
+    sc.trace[1] = &t1;
+    analyse_pair(c, r, o, &hb, &bwd, &hf, &fwd, &sc, false);
+    auto payload = pack_vector_frame(o, hb ? &bwd : nullptr, hf ? &fwd : nullptr);
+    *out_len = payload.size();
+    *n0 = t0.size();
+    *n1 = t1.size();
+    if (payload.size() > out_cap || t0.size() > cap0 || t1.size() > cap1) return -1;
+    std::memcpy(out, payload.data(), payload.size());
+    if (!t0.empty()) std::memcpy(trace0, t0.data(), t0.size() * sizeof(BlockTrace));
+    if (!t1.empty()) std::memcpy(trace1, t1.data(), t1.size() * sizeof(BlockTrace));
+    return 0;
+}
+
 // Single candidate cost for focused tests: (luma, chroma) by the reference's path selection.
 int oracle_candidate_cost(int width, int height, int pel, int super_levels, const uint8_t* cy, int cys,
                           const uint8_t* cu, int cus, const uint8_t* cv, int cvs, const uint8_t* ry, int rys,

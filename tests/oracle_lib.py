@@ -227,3 +227,67 @@ class SmoothClip:
                                     C.byref(algo))
         assert rc >= 0, rc
         return (Y, U, V), rc, algo.value
+
+# Per-block trace of the oracle's level searches (BlockTrace in oracle/analyse.h, 64 int32 words).
+TRACE_FIELDS = ["level", "dir", "bx", "by", "px", "py", "bw", "bh", "edge", "smallest",
+                "pred_dx", "pred_dy", "pred_score", "has_prev", "prev_dx", "prev_dy", "gx", "gy", "nseeds"] + \
+               [f"seed{i}" for i in range(16)] + \
+               ["lambda", "radius", "stype", "stage_mvx", "stage_mvy", "stage_sad", "mpred_x", "mpred_y",
+                "found", "bad_before", "bad_limit", "expanded", "out_dx", "out_dy", "out_score", "out_luma"]
+TRACE_IDX = {k: i for i, k in enumerate(TRACE_FIELDS)}
+L.oracle_analyse_trace.argtypes = [C.c_char_p, _sz, C.c_int64, _i, _i] + [u8p, _i] * 6 + [
+    u8p, _sz, C.POINTER(_sz), C.POINTER(C.c_int32), _sz, C.POINTER(_sz), C.POINTER(C.c_int32), _sz, C.POINTER(_sz)]
+
+
+class Trace:
+    """Rows of BlockTrace for one direction, in visit order.  `t[k]` is column k as
+    int64 (scores are read back as unsigned 32-bit)."""
+
+    def __init__(self, rows: np.ndarray):
+        self.rows = rows.astype(np.int64)
+        for k in ("pred_score", "stage_sad", "found", "out_score"):
+            c = TRACE_IDX[k]
+            self.rows[:, c] &= 0xFFFFFFFF
+
+    def __len__(self):
+        return len(self.rows)
+
+    def __getitem__(self, k):
+        return self.rows[:, TRACE_IDX[k]]
+
+    def level(self, lv):
+        return Trace.__new__(Trace)._set(self.rows[self.rows[:, 0] == lv])
+
+    def _set(self, rows):
+        self.rows = rows
+        return self
+
+    def rec(self, i):
+        return {k: int(self.rows[i, j]) for j, k in enumerate(TRACE_FIELDS)}
+
+
+def analyse_trace(opt: str | None, sdata: int, width: int, height: int, cur, ref):
+    """get_frame_analyse with a trace: (payload, Trace dir 0 (cur->ref), Trace dir 1 (ref->cur))."""
+    b = (opt or "").encode()
+    o, err = analyse_opts(opt, sdata, width, height)
+    if o is None:
+        raise ValueError(err)
+    out = np.zeros(o["payload_len"] + 64, np.uint8)
+    n = _sz()
+    keep = [np.ascontiguousarray(p) for p in list(cur) + list(ref)]
+    args = []
+    for p in keep:
+        args += [_p(p), p.shape[1]]
+    cap = 1
+    while True:
+        t0 = np.zeros((cap, 64), np.int32)
+        t1 = np.zeros((cap, 64), np.int32)
+        n0, n1 = _sz(), _sz()
+        rc = L.oracle_analyse_trace(b, len(b), sdata, width, height, *args, _p(out), out.size, C.byref(n),
+                                    t0.ctypes.data_as(C.POINTER(C.c_int32)), cap, C.byref(n0),
+                                    t1.ctypes.data_as(C.POINTER(C.c_int32)), cap, C.byref(n1))
+        if rc == 0:
+            break
+        assert rc == -1, rc
+        cap = max(n0.value, n1.value, 1)
+    return out[:n.value].copy(), Trace(t0[:n0.value]), Trace(t1[:n1.value])
