@@ -118,9 +118,12 @@ __device__ __forceinline__ int clamp_to_frame(int v, int pos, int block, int fra
     return v;
 }
 
-__device__ uint8_t scale_magnitude(uint32_t mag, int area, double scale, double expo) {  // renderer.rs:3245-3259
-    const double base = 4.0 * (double)mag * scale / (double)area;
-    const double v = (expo == 1.0 ? base : pow(base, expo)) * 255.0;
+// scale_magnitude (renderer.rs:3245-3259).  With the exponent 1 the power is the base
+// itself (pow(x, 1) == x), so the IEEE products and quotient below are the whole
+// computation; any other exponent reads the host-built table (magnitudes are 24-bit).
+__device__ uint8_t scale_magnitude(uint32_t mag, int area, const SmoothGeom& g) {
+    if (g.mag_lut) return g.mag_lut[mag & 0xFFFFFFu];
+    const double v = __dmul_rn(__ddiv_rn(__dmul_rn(__dmul_rn(4.0, (double)mag), g.area_scale), (double)area), 255.0);
     if (!isfinite(v) || v <= 0.0) return 0;
     if (v >= 255.0) return 255;
     return (uint8_t)v;
@@ -161,8 +164,8 @@ __global__ void prep_planes_kernel(const PrepJob* __restrict__ jobs, const Smoot
             else { cx = 0; cy = g.gh - 1; }
             Dec ma, mb;
             decode_sets(g, j.payload, cy * g.gw + cx, ma, mb);
-            j.mag[0][i] = scale_magnitude(ma.score, area, g.area_scale, g.area_sharp);
-            j.mag[1][i] = scale_magnitude(mb.score, area, g.area_scale, g.area_sharp);
+            j.mag[0][i] = scale_magnitude(ma.score, area, g);
+            j.mag[1][i] = scale_magnitude(mb.score, area, g);
         }
     }
 }
@@ -225,28 +228,12 @@ __global__ void cov_clear_kernel(const FrameJob* __restrict__ jobs, const Smooth
     }
 }
 
-// TruncDiv (renderer.rs:3149-3184): |v| < bound uses the 40-bit reciprocal, else division.
-struct TDiv {
-    int d;
-    unsigned long long magic, bound;
-    __device__ explicit TDiv(int div) : d(div), magic(0), bound(0) {
-        if (div > 0) {
-            const unsigned long long du = (unsigned long long)div;
-            magic = ((1ull << 40) + du - 1) / du;
-            unsigned long long b = (1ull << 40) / du, lim = ~0ull / magic;
-            b = b < lim ? b : lim;
-            bound = b < (1ull << 31) ? b : (1ull << 31);
-        }
-    }
-    __device__ int div(int v) const {
-        const unsigned long long a = (unsigned long long)(v < 0 ? -(long long)v : (long long)v);
-        if (a < bound) {
-            const int q = (int)((a * magic) >> 40);
-            return v < 0 ? -q : q;
-        }
-        return d == 0 ? 0 : v / d;
-    }
-};
+// TruncDiv (renderer.rs:3149-3184).  Its 40-bit reciprocal m = ceil(2^40/d) is used only
+// for |v| < 2^40/d, where |v|*m/2^40 = |v|/d + e with 0 <= e < |v|/2^40 < 1/d, so the
+// floor of that product is the floor of |v|/d; every other case is Rust's `/`.  So it
+// is truncating division for every divisor other than 0 (which gives 0), and the
+// device divides directly.
+__device__ __forceinline__ int trunc_div(int v, int d) { return d == 0 ? 0 : v / d; }
 
 // splat_coverage (renderer.rs:2945-3049): mask 0 splats set b at threshold 256-phase,
 // mask 1 splats set a at threshold phase.  Integer atomics: the sums do not depend on order.
@@ -259,7 +246,6 @@ __global__ void cov_splat_kernel(const FrameJob* __restrict__ jobs, const FrameP
     const int x_step = g.vbw - g.vox, y_step = g.vbh - g.voy;
     const int denom = max(g.marker, 1) << 8;
     if (x_step == 0 || y_step == 0 || denom == 0 || g.mw <= 0 || g.mh <= 0) return;
-    const TDiv dd(denom), xd(x_step), yd(y_step);
     const int x_bias = 1 - x_step, y_bias = 1 - y_step;
     const int stride = g.mw + 2;
     const int total = 2 * g.count;
@@ -269,10 +255,10 @@ __global__ void cov_splat_kernel(const FrameJob* __restrict__ jobs, const FrameP
         const int16_t* vec = m == 0 ? j.vec_b : j.vec_a;
         const int threshold = m == 0 ? 256 - p.phase : p.phase;
         int32_t* acc = m == 0 ? j.cov_acc[0] : j.cov_acc[1];  // (an index by lane would use scratch)
-        const int dx = dd.div(threshold * (int)vec[2 * i]), dy = dd.div(threshold * (int)vec[2 * i + 1]);
+        const int dx = trunc_div(threshold * (int)vec[2 * i], denom), dy = trunc_div(threshold * (int)vec[2 * i + 1], denom);
         const int sx = x * x_step + dx, sy = y_step * y + dy;
-        const int left = xd.div(sx + (x_bias & (sx >> 31)));
-        const int top = yd.div(sy + (y_bias & (sy >> 31)));
+        const int left = trunc_div(sx + (x_bias & (sx >> 31)), x_step);
+        const int top = trunc_div(sy + (y_bias & (sy >> 31)), y_step);
         const int right = left + 1, bottom = top + 1;
         const int next_x = right * x_step;
         const int lw = next_x - sx, tw = y_step * bottom - sy;

@@ -26,6 +26,9 @@ struct SmoothGeom {
     // options
     int32_t algo_req, force13, mask_area, mask_cover;
     double area_scale, area_sharp;
+    // scale_magnitude for every 24-bit magnitude, built on the host with the host libm's
+    // pow (as the reference computes it), when area_sharp != 1; null otherwise.
+    const uint8_t* mag_lut;
     int32_t lim_zero, lim_m1, lim_m2, lim_scene, lim_blocks;
     int32_t nframes;
     uint8_t luma_lut[512];           // luma_byte((a + b) / denom, gamma) for a + b in 0..510

@@ -4,6 +4,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
+#include <vector>
 
 #include "options.h"
 #include "smooth_dev.h"
@@ -163,6 +165,52 @@ const char* parse_opts(const Opt* op, int64_t fps_num, int64_t fps_den, int vbw,
     return nullptr;
 }
 
+// Options::padding (options.rs:256-277, pad_x/pad_y :788-796): the border the light.*
+// options add around the output, per side.
+void light_padding(const Opt* op, int width, int height, int* pad_x, int* pad_y) {
+    *pad_x = *pad_y = 0;
+    if (width <= 0 || height <= 0) return;
+    double f;
+    auto gf = [&](std::initializer_list<const char*> k, double d) { return (op && op->get_float(k, &f)) ? f : d; };
+    const double w = (double)width, h = (double)height, src_aspect = w / h;
+    const double sar = gf({"light", "sar"}, 1.0);
+    double aspect = gf({"light", "aspect"}, 0.0) / sar;
+    if (aspect < 0.01) aspect = src_aspect;  // (a NaN aspect stays NaN, as in the reference)
+    const double zoom = std::max(gf({"light", "zoom"}, 0.0), 0.0);
+    int tw = width, th = height;
+    if (std::fabs(aspect - src_aspect) <= 0.01) {
+    } else if (src_aspect <= aspect) {
+        tw = trunc_i32(aspect * h);
+    } else {
+        th = trunc_i32(w / aspect);
+    }
+    const int x = (int)((int64_t)trunc_i32((double)tw * zoom / 100.0) + tw - width);
+    const int y = (int)((int64_t)trunc_i32((double)th * zoom / 100.0) + th - height);
+    const int hx = x / 2, hy = y / 2;
+    *pad_x = (hx + 3 * (hx + 1 < 0 ? 1 : 0) + 1) & ~3;
+    *pad_y = (hy + (hy + 1 < 0 ? 1 : 0) + 1) & ~1;
+}
+
+// scale_magnitude (renderer.rs:3245-3259) for every 24-bit magnitude, evaluated with this
+// host's libm pow in the reference's operation order.  The reference computes the mask on
+// the CPU with the same expression, so the table is that computation, not an
+// approximation of it; 2^24 entries cover every score a vector payload can carry.
+std::vector<uint8_t> magnitude_table(int area, double scale, double expo) {
+    const size_t n = size_t(1) << 24;
+    std::vector<uint8_t> t(n);
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned w = 0; w < hw; w++)
+        th.emplace_back([&, w] {
+            for (size_t m = n * w / hw; m < n * (w + 1) / hw; m++) {
+                const double v = std::pow(4.0 * (double)(uint32_t)m * scale / (double)area, expo) * 255.0;
+                t[m] = (!std::isfinite(v) || v <= 0.0) ? 0 : (v >= 255.0 ? 255 : (uint8_t)v);
+            }
+        });
+    for (auto& x : th) x.join();
+    return t;
+}
+
 }  // namespace
 }  // namespace svpf
 
@@ -177,6 +225,7 @@ struct svpf_smooth {
     int32_t vdata[16];
     SmoothGeom geom;
     SmoothGeom* d_geom = nullptr;
+    uint8_t* d_mag_lut = nullptr;
     int count, mw, mh, mc;
     std::vector<svpf_frame> src, sup;
     std::vector<uint8_t*> vec;
@@ -300,6 +349,14 @@ int svpf_smooth_create(svpf_ctx* ctx, const char* opt, size_t opt_len, int32_t w
         delete s;
         return reject("SVSmoothFps: 'cubic' mode isn't available with CPU rendering");
     }
+    int pad_x, pad_y;
+    light_padding(have ? &o : nullptr, width, height, &pad_x, &pad_y);
+    if (pad_x != 0 || pad_y != 0) {
+        // The reference widens the output by 2*padding and draws its light border there
+        // (core.rs:603-613, 1866-1897, light.rs); this build renders only unpadded output.
+        delete s;
+        return reject("SVSmoothFps: light.aspect/light.zoom padding is not supported by this build");
+    }
     s->timing = s->opt.timing(fps_num, fps_den);
     s->width = width;
     s->height = height;
@@ -394,9 +451,17 @@ int svpf_smooth_create(svpf_ctx* ctx, const char* opt, size_t opt_len, int32_t w
     s->mh = g.mh;
     s->mc = g.mw * g.mh;
     hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess && g.mask_area && g.area_sharp != 1.0) {
+        const std::vector<uint8_t> t = magnitude_table(area, g.area_scale, g.area_sharp);
+        e = hipMalloc(&s->d_mag_lut, t.size());
+        if (e == hipSuccess) e = hipMemcpy(s->d_mag_lut, t.data(), t.size(), hipMemcpyHostToDevice);
+        g.mag_lut = s->d_mag_lut;
+    }
     if (e == hipSuccess) e = hipMalloc(&s->d_geom, sizeof(SmoothGeom));
     if (e == hipSuccess) e = hipMemcpy(s->d_geom, &g, sizeof g, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
+        if (s->d_mag_lut) (void)hipFree(s->d_mag_lut);
+        if (s->d_geom) (void)hipFree(s->d_geom);
         delete s;
         return fail(SVPF_ERR_HIP, std::string("svpf_smooth_create: ") + hipGetErrorString(e));
     }
@@ -417,6 +482,7 @@ void svpf_smooth_destroy(svpf_smooth* s) {
         if (p) (void)hipFree(p);
     if (s->scratch) (void)hipFree(s->scratch);
     if (s->d_geom) (void)hipFree(s->d_geom);
+    if (s->d_mag_lut) (void)hipFree(s->d_mag_lut);
     delete s;
 }
 

@@ -319,3 +319,61 @@ int oracle_smooth_render(const char* opt, size_t len, int width, int height, int
 }
 
 }  // extern "C"
+
+extern "C" {
+
+int oracle_truncdiv(int divisor, int value) { return truncdiv(divisor, value); }
+
+// Vector sets as (dx, dy, magnitude) triples, grid_w * grid_h each.
+static std::vector<RVector> rvecs(const int32_t* t, int n) {
+    std::vector<RVector> v((size_t)n);
+    for (int i = 0; i < n; i++) v[(size_t)i] = RVector{(int16_t)t[3 * i], (int16_t)t[3 * i + 1], (uint32_t)t[3 * i + 2]};
+    return v;
+}
+
+// coverage_mask (renderer.rs:1738-1763) of set `which` (it splats the opposite set).
+int oracle_coverage(int block_w, int block_h, int scale_shift_base, int origin_x, int origin_y, int grid_w, int grid_h,
+                    const int32_t* a, const int32_t* b, int which, int width, int height, int strength, int threshold,
+                    uint8_t* out) {
+    auto va = rvecs(a, grid_w * grid_h), vb = rvecs(b, grid_w * grid_h);
+    VectorContext ctx{block_w, block_h, scale_shift_base, 0, 0, origin_x, origin_y, (size_t)grid_w, (size_t)grid_h,
+                      false, &va, &vb};
+    std::vector<uint8_t> m((size_t)width * height);
+    coverage_mask(ctx, which, m, (size_t)width, (size_t)height, strength, threshold);
+    std::memcpy(out, m.data(), m.size());
+    return 0;
+}
+
+// magnitude_mask (renderer.rs:1647-1678) of set `which`.
+int oracle_magnitude(int block_w, int block_h, int grid_w, int grid_h, const int32_t* a, const int32_t* b, int which,
+                     int width, int height, double scale, double exponent, uint8_t* out) {
+    auto va = rvecs(a, grid_w * grid_h), vb = rvecs(b, grid_w * grid_h);
+    VectorContext ctx{block_w, block_h, 1, 0, 0, 0, 0, (size_t)grid_w, (size_t)grid_h, false, &va, &vb};
+    std::vector<uint8_t> m((size_t)width * height);
+    magnitude_mask(ctx, which, m, (size_t)width, (size_t)height, scale, exponent);
+    std::memcpy(out, m.data(), m.size());
+    return 0;
+}
+
+int oracle_luma_byte(double v, double gamma) { return luma_byte(v, gamma); }
+
+// analyze_scene over decoded vectors given as (score, luma) pairs; prev may be null.
+int oracle_scene_class(const int32_t* prev, const int32_t* cur, int grid_w, int grid_h, int marker, double gamma,
+                       int64_t zero, int64_t m1, int64_t m2, int64_t scene, int64_t blocks) {
+    const int n = grid_w * grid_h;
+    std::vector<DVec> p((size_t)n), c((size_t)n);
+    for (int i = 0; i < n; i++) {
+        c[(size_t)i] = DVec{0, 0, (uint32_t)cur[2 * i], (uint8_t)cur[2 * i + 1]};
+        if (prev) p[(size_t)i] = DVec{0, 0, (uint32_t)prev[2 * i], (uint8_t)prev[2 * i + 1]};
+    }
+    VectorData vd{};
+    vd.grid_w = grid_w;
+    vd.grid_h = grid_h;
+    vd.marker = marker;
+    SmoothOptions o;
+    o.scene_luma = gamma;
+    o.lim_zero = zero; o.lim_m1 = m1; o.lim_m2 = m2; o.lim_scene = scene; o.lim_blocks = blocks;
+    return scene_class(prev ? &p : nullptr, &c, vd, o);
+}
+
+}  // extern "C"

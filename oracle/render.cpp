@@ -796,4 +796,6 @@ int Timing::scale_frame_count(int frames) const {  // frame_math.rs:128-135
     return clamp_i32_64(sat_mul64(frames - 1, frame_num) / frame_den + 1);
 }
 
+int truncdiv(int divisor, int value) { return TruncDiv(divisor).div(value); }  // renderer.rs:3149-3184
+
 }  // namespace oracle

@@ -56,6 +56,7 @@ void coverage_mask(const VectorContext& ctx, int which, std::vector<uint8_t>& ou
 void magnitude_mask(const VectorContext& ctx, int which, std::vector<uint8_t>& out, size_t w, size_t h, double scale,
                     double exponent);  // renderer.rs:1647-1678
 std::vector<uint8_t> max_mask(const std::vector<uint8_t>& a, const std::vector<uint8_t>& b);
+int truncdiv(int divisor, int value);  // TruncDiv::new(divisor).div(value), renderer.rs:3149-3184
 
 class CpuRenderer {  // renderer.rs:63-1638
 public:
@@ -153,6 +154,10 @@ struct SmoothClip {
 };
 
 int smooth_output_frames(const SmoothClip& c);
+uint8_t luma_byte(double v, double gamma);  // metadata.rs:523-526
+// analyze_scene (metadata.rs:349-362): class 0..3 of `cur` (luma from prev + cur).
+int scene_class(const std::vector<DVec>* prev, const std::vector<DVec>* cur, const VectorData& vd,
+                const SmoothOptions& o);
 // Renders output frame k into dst (3 planes, strides width, width/2).  Returns a code of
 // what happened: 0 copied source n, 1 copied source n+1, 2 rendered; *algo_out = algorithm used.
 int smooth_render_frame(const SmoothClip& c, int k, uint8_t* dst[3], int* algo_out);

@@ -22,6 +22,8 @@ int32_t trunc_i32(double v) {
 }
 int32_t sat_i32(int64_t v) { return (int32_t)std::min<int64_t>(std::max<int64_t>(v, INT32_MIN), INT32_MAX); }
 
+}  // namespace
+
 uint8_t luma_byte(double v, double gamma) {  // metadata.rs:523-526
     int s = trunc_i32(std::pow(v, gamma) * 255.0);
     return (uint8_t)(std::max(s, 20) & 0xFF);
@@ -58,6 +60,8 @@ int scene_class(const std::vector<DVec>* prev, const std::vector<DVec>* cur, con
     int cls = scene_n >= required ? 3 : (high >= required ? 2 : (mid >= required ? 1 : 0));
     return std::max(cls, 0);
 }
+
+namespace {
 
 std::vector<RVector> renderer_vectors(const std::vector<DVec>& v) {  // core.rs:3141-3150
     std::vector<RVector> r(v.size());

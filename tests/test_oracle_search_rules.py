@@ -107,7 +107,8 @@ def exact_lambda(lam: int, plevel: float, levels: int, level: int, pel: int) -> 
 def adapt_lambda(nl: int, lsad: int, score: int) -> int:
     l = float(max(lsad, 1))
     half = -1.0 if score == MAXU else float(score >> 1)
-    t = l / (l + half)
+    den = l + half
+    t = l / den if den != 0.0 else math.inf        # IEEE: a positive over +0 is +inf
     v = nl * t * t
     v = 21474836.47 if math.isnan(v) else min(v, 21474836.47)   # f64::min ignores a NaN operand
     return f2i(v)

@@ -146,9 +146,19 @@ def test_render_cfg3_settings_small(gpu_ctx, oracle):
     "{rate:{num:3,den:1},algo:22,scene:{mode:2,blend:true}}",
     "{rate:{num:5,den:2},algo:21,scene:{mode:0,limits:{zero:0,m1:10,m2:20,scene:40000,blocks:50}}}",
     "{rate:{num:3,den:2},algo:13,scene:{mode:0}}",
+    "{rate:{num:5,den:2},algo:23,scene:{mode:0},mask:{area:100,area_sharp:0.5}}",
+    "{rate:{num:5,den:2},algo:13,scene:{mode:0},mask:{area:200,area_sharp:2.0}}",
+    "{rate:{num:2,den:1},algo:21,scene:{mode:0},mask:{area:40,area_sharp:0.37}}",
 ])
 def test_render_options(gpu_ctx, oracle, pipe1, opt):
     _check(gpu_ctx, oracle, pipe1, opt)
+
+
+def test_render_cfg2_magnitude_exponents(gpu_ctx, oracle, pipe2):
+    # mask.area_sharp != 1: the magnitude mask reads the host-built pow table
+    for sharp in (0.5, 2.0):
+        _check(gpu_ctx, oracle, pipe2, "{rate:{num:5,den:2},algo:23,mask:{area:100,area_sharp:%s},scene:{mode:0}}"
+               % sharp, fps=(24, 1))
 
 
 def test_render_cfg2_vectors_all_algos(gpu_ctx, oracle, pipe2):

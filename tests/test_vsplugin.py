@@ -87,6 +87,12 @@ def test_create_checks_and_clip_info(core):
                     opt=b"{rate:{num:2,den:1},algo:7}")
     with pytest.raises(vsfake.VSError, match="SVSmoothFps: invalid vectors stream"):
         core.invoke("svp2", "SmoothFps", clip=src, super=sup, sdata=sdata, vectors=vec, opt=ROPT)
+    # light.* options that widen the output (Options::padding, options.rs:256-277) are
+    # rejected instead of rendering a frame of another size than the reference's
+    for light in (b"light:{aspect:2.35}", b"light:{zoom:10}", b"light:{aspect:1.0,sar:0.5}"):
+        with pytest.raises(vsfake.VSError, match="light.aspect/light.zoom padding is not supported"):
+            core.invoke("svp2", "SmoothFps", clip=src, super=sup, sdata=sdata, vectors=vec, vdata=vdata_addr,
+                        opt=b"{rate:{num:2,den:1}," + light + b"}")
     with pytest.raises(vsfake.VSError, match="SVSmoothFps: CUDA is not availabe"):
         core.invoke("svp2", "SmoothFps_NVOF", clip=src, opt=b"{}")
 
