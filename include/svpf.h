@@ -149,6 +149,18 @@ int svpf_analyse_batch(svpf_ctx* ctx, const svpf_analyse_params* p, int32_t n, c
 int svpf_analyse_multi(svpf_ctx* ctx, int32_t n, const svpf_analyse_params* const* params, const svpf_frame* d_cur,
                        const svpf_frame* d_ref, uint8_t* const* d_payload);
 
+/* No counterpart in the reference.  svpf_analyse_pair for callers that cannot
+ * allocate device memory (serving loops, harnesses that forbid hipMalloc): the same
+ * arguments and payload, with the search's workspace, sort storage and descriptors
+ * carved from a fixed 128 MiB device array per GPU instead of the context's
+ * grow-only buffers.  A pair that needs more (beyond about one 2560x1440 pair with
+ * 8x8 blocks and overlap 4) fails with SVPF_ERR_ARG and says how much it needs.
+ * Every call on one GPU shares that array: calls that overlap on a GPU (two threads
+ * or two contexts) race and must be serialised by the caller.  Like the other entry
+ * points it synchronises the context's stream while it runs. */
+int svpf_analyse_pair_noalloc(svpf_ctx* ctx, const svpf_analyse_params* p, const svpf_frame* d_cur,
+                              const svpf_frame* d_ref, uint8_t* d_payload);
+
 /* ------------------------------------------------------------------ SmoothFps */
 typedef struct svpf_smooth svpf_smooth;
 

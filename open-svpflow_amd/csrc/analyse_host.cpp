@@ -162,15 +162,18 @@ constexpr int kNotConverged = 1;
 // chain's searched block is a 4x4 or 8x8 square, and sweep 0 (every block of the level)
 // runs the tile-plan kernel, then the general one over the blocks it passed on.
 int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total_blocks, const SweepState& st,
-               bool scan, int level, bool fast) {
+               bool scan, int level, bool fast, bool no_alloc) {
     const size_t words = (size_t)kMaxSweeps + 2;
-    if (ctx->h_count_cap < words) {
+    uint32_t local_count = 0;  // no_alloc: read back through pageable memory (a synchronous copy)
+    uint32_t* h_count = no_alloc ? &local_count : nullptr;
+    if (!no_alloc && ctx->h_count_cap < words) {
         if (ctx->h_count) (void)hipHostFree(ctx->h_count);
         ctx->h_count = nullptr;
         ctx->h_count_cap = 0;
         SVPF_HIP(hipHostMalloc((void**)&ctx->h_count, 4 * words, hipHostMallocDefault));
         ctx->h_count_cap = words;
     }
+    if (!no_alloc) h_count = ctx->h_count;
     const int grid = (int)std::min<size_t>(std::max<size_t>(total_blocks, 1), 4096);
     SVPF_HIP(launch_sweep0(dch, nch, max_n, st, ctx->stream, level, fast));
     if (fast) SVPF_HIP(launch_deferred(dch, grid, st, 0, ctx->stream));
@@ -184,9 +187,9 @@ int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total
             if (scan) SVPF_HIP(launch_sweep_scan(dch, nch, st, s + k, ctx->stream));
         }
         s += chunk;
-        SVPF_HIP(hipMemcpyAsync(ctx->h_count, st.counts + s, 4, hipMemcpyDeviceToHost, ctx->stream));
+        SVPF_HIP(hipMemcpyAsync(h_count, st.counts + s, 4, hipMemcpyDeviceToHost, ctx->stream));
         SVPF_HIP(hipStreamSynchronize(ctx->stream));
-        if (*ctx->h_count == 0) {
+        if (*h_count == 0) {
             if (ctx->prof)  // statistics for the benchmark, added on the device
                 SVPF_HIP(launch_sweep_stats(st, s, (long long)total_blocks,
                                             ctx->d_sweep_stats + 4 * std::min(level, 15), ctx->stream));
@@ -199,7 +202,7 @@ int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total
 }  // namespace
 
 int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, const svpf_frame* d_cur,
-                const svpf_frame* d_ref, uint8_t* const* d_payload) {
+                const svpf_frame* d_ref, uint8_t* const* d_payload, const ExtWorkspace* ext) {
     if (n <= 0) return SVPF_OK;
     // ---- per-clip plans (clips are identified by their params pointer)
     std::vector<ClipPlan> clips;
@@ -367,16 +370,6 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
         sort_vals_in[l] = work.take(4 * tot + 4);
         sort_vals_out[l] = work.take(4 * tot + 4);
     }
-    if (work.off > ctx->work_cap) {
-        SVPF_HIP(hipStreamSynchronize(ctx->stream));
-        if (ctx->d_work) (void)hipFree(ctx->d_work);
-        ctx->d_work = nullptr;
-        ctx->work_cap = 0;
-        size_t cap = work.off + work.off / 4;
-        hipError_t e = hipMalloc(&ctx->d_work, cap);
-        if (e != hipSuccess) return fail(SVPF_ERR_NOMEM, "svpf_analyse: workspace allocation failed");
-        ctx->work_cap = cap;
-    }
     // sort temp storage (max over levels)
     size_t sort_tmp = 0;
     for (int l = 1; l < max_levels; l++) {
@@ -386,16 +379,47 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
                                 nullptr, nullptr, ctx->stream));
         sort_tmp = std::max(sort_tmp, bytes);
     }
-    if (sort_tmp > ctx->sort_tmp_cap) {
-        SVPF_HIP(hipStreamSynchronize(ctx->stream));
-        if (ctx->d_sort_tmp) (void)hipFree(ctx->d_sort_tmp);
-        ctx->d_sort_tmp = nullptr;
-        ctx->sort_tmp_cap = 0;
-        hipError_t e = hipMalloc(&ctx->d_sort_tmp, sort_tmp);
-        if (e != hipSuccess) return fail(SVPF_ERR_NOMEM, "svpf_analyse: sort workspace allocation failed");
-        ctx->sort_tmp_cap = sort_tmp;
+    uint8_t* W = nullptr;
+    void* sort_ptr = nullptr;
+    size_t sort_cap = 0;
+    void* d_blob = nullptr;
+    if (ext) {
+        // one external buffer: descriptor blob, then the workspace, then the sort storage
+        const size_t work_at = (blob.off + 255) / 256 * 256;
+        const size_t sort_at = (work_at + work.off + 255) / 256 * 256;
+        const size_t need = sort_at + sort_tmp;
+        if (need > ext->cap)
+            return fail(SVPF_ERR_ARG, "svpf_analyse_pair_noalloc: this pair needs " + std::to_string(need) +
+                                          " bytes of scratch memory, more than its maximum of " +
+                                          std::to_string(ext->cap));
+        d_blob = ext->base;
+        W = ext->base + work_at;
+        sort_ptr = ext->base + sort_at;
+        sort_cap = sort_tmp;
+    } else {
+        if (work.off > ctx->work_cap) {
+            SVPF_HIP(hipStreamSynchronize(ctx->stream));
+            if (ctx->d_work) (void)hipFree(ctx->d_work);
+            ctx->d_work = nullptr;
+            ctx->work_cap = 0;
+            size_t cap = work.off + work.off / 4;
+            hipError_t e = hipMalloc(&ctx->d_work, cap);
+            if (e != hipSuccess) return fail(SVPF_ERR_NOMEM, "svpf_analyse: workspace allocation failed");
+            ctx->work_cap = cap;
+        }
+        if (sort_tmp > ctx->sort_tmp_cap) {
+            SVPF_HIP(hipStreamSynchronize(ctx->stream));
+            if (ctx->d_sort_tmp) (void)hipFree(ctx->d_sort_tmp);
+            ctx->d_sort_tmp = nullptr;
+            ctx->sort_tmp_cap = 0;
+            hipError_t e = hipMalloc(&ctx->d_sort_tmp, sort_tmp);
+            if (e != hipSuccess) return fail(SVPF_ERR_NOMEM, "svpf_analyse: sort workspace allocation failed");
+            ctx->sort_tmp_cap = sort_tmp;
+        }
+        W = (uint8_t*)ctx->d_work;
+        sort_ptr = ctx->d_sort_tmp;
+        sort_cap = ctx->sort_tmp_cap;
     }
-    uint8_t* W = (uint8_t*)ctx->d_work;
 
     // ---- fill static parts of the blob
     for (auto& c : clips) {
@@ -411,12 +435,14 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
     }
     // Device addresses of blob entries are only known after the upload; pointers into
     // the blob are written as offsets first and patched below.
-    void* d_blob = nullptr;
     auto blob_ptr = [&](size_t off) -> uint8_t* { return (uint8_t*)d_blob + off; };
     // The upload needs the final content, which includes device pointers into the blob:
     // reserve the device buffer first by uploading the current (partial) blob.
-    int rc = svpf_upload_desc(ctx, host.data(), host.size(), &d_blob);
-    if (rc) return rc;
+    int rc = SVPF_OK;
+    if (!ext) {
+        rc = svpf_upload_desc(ctx, host.data(), host.size(), &d_blob);
+        if (rc) return rc;
+    }
 
     // ---- per-step tables
     auto chain_alive = [&](int ci) {
@@ -632,10 +658,17 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
         // clips with different denominators: pack per clip below (one launch per clip)
     }
     // final upload with every device pointer in place
-    void* first_blob = d_blob;
-    rc = svpf_upload_desc(ctx, host.data(), host.size(), &d_blob);
-    if (rc) return rc;
-    if (d_blob != first_blob) return fail(SVPF_ERR_STATE, "svpf_analyse: descriptor buffer moved");
+    if (ext) {
+        // a copy from pageable memory, ordered after the previous call's kernels on the
+        // stream; the wait lets `host` go out of scope whatever the runtime's staging
+        SVPF_HIP(hipMemcpyAsync(d_blob, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream));
+        SVPF_HIP(hipStreamSynchronize(ctx->stream));
+    } else {
+        void* first_blob = d_blob;
+        rc = svpf_upload_desc(ctx, host.data(), host.size(), &d_blob);
+        if (rc) return rc;
+        if (d_blob != first_blob) return fail(SVPF_ERR_STATE, "svpf_analyse: descriptor buffer moved");
+    }
 
     // ---- launches
     if (ninit) SVPF_HIP(launch_interp(blob_ptr(init_off), ninit, max_init_n, ctx->stream));
@@ -646,10 +679,10 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
         }
         if (nsort_step[l]) {
             SVPF_HIP(launch_sort_keys(blob_ptr(sj_off[l]), nsort_step[l], max_sort_n[l], ctx->stream));
-            size_t bytes = ctx->sort_tmp_cap;
+            size_t bytes = sort_cap;
             const int* segp = (const int*)blob_ptr(seg_off[l]);
             const int total = (int)sort_total[l + 1];
-            SVPF_HIP(segmented_sort(ctx->d_sort_tmp, &bytes, (const uint32_t*)(W + sort_keys_in[l + 1]),
+            SVPF_HIP(segmented_sort(sort_ptr, &bytes, (const uint32_t*)(W + sort_keys_in[l + 1]),
                                     (uint32_t*)(W + sort_keys_out[l + 1]), (const uint32_t*)(W + sort_vals_in[l + 1]),
                                     (uint32_t*)(W + sort_vals_out[l + 1]), total, nsort_step[l], segp,
                                     segp + nchains + 1, ctx->stream));
@@ -677,7 +710,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
             SVPF_HIP(launch_sweep_init(dch, nch_step[l], level_max_n[l], st, ctx->stream));
             int rc2 = seq_step[l] ? SVPF_OK
                                   : run_sweeps(ctx, dch, nch_step[l], level_max_n[l], level_blocks[l], st, bad_step[l],
-                                               l, plan_step[l]);
+                                               l, plan_step[l], ext != nullptr);
             if (rc2 == kNotConverged || seq_step[l])
                 SVPF_HIP(launch_search_level(dch, nch_step[l], ctx->stream));
             else if (rc2)

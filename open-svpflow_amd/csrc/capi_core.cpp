@@ -156,6 +156,12 @@ int svpf_create(int device, svpf_ctx** out) {
         delete c;
         return fail(SVPF_ERR_HIP, std::string("svpf_create: ") + hipGetErrorString(e));
     }
+    // device address of svpf_analyse_pair_noalloc's scratch array on this device
+    const int rc = noalloc_scratch_init(device);
+    if (rc != SVPF_OK) {
+        svpf_destroy(c);
+        return rc;
+    }
     *out = c;
     return SVPF_OK;
 }

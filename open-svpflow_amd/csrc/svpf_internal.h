@@ -89,6 +89,19 @@ int svpf_upload_desc(svpf_ctx* ctx, const void* data, size_t bytes, void** d_out
 
 namespace svpf {
 
+// Analyse over caller-provided device memory instead of the context's grow-only
+// buffers: the descriptor blob, the workspace and the sort storage are carved from
+// [base, base + cap); nothing is allocated.
+struct ExtWorkspace {
+    uint8_t* base;
+    size_t cap;
+};
+int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, const svpf_frame* d_cur,
+                const svpf_frame* d_ref, uint8_t* const* d_payload, const ExtWorkspace* ext = nullptr);
+// Resolves the allocation-free entry point's scratch array on `device` (which must be
+// current); once per device (analyse_noalloc.hip).
+int noalloc_scratch_init(int device);
+
 // Kernel launchers (super.hip)
 hipError_t launch_super_level0(const SuperGeom& g, const Planes* src, const Planes* dst, int nframes,
                                int max_stride, hipStream_t s);
