@@ -149,6 +149,12 @@ int svpf_analyse_batch(svpf_ctx* ctx, const svpf_analyse_params* p, int32_t n, c
 int svpf_analyse_multi(svpf_ctx* ctx, int32_t n, const svpf_analyse_params* const* params, const svpf_frame* d_cur,
                        const svpf_frame* d_ref, uint8_t* const* d_payload);
 
+/* Runtime knob, no reference counterpart: the number of parallel sweeps a pyramid
+ * level may take before the context falls back to the sequential walk of the
+ * reference's visit order (one wavefront per frame pair and direction).  Both give
+ * the same vectors; 0 restores the default of 4096, the maximum. */
+int svpf_analyse_set_max_sweeps(svpf_ctx* ctx, int32_t max_sweeps);
+
 /* No counterpart in the reference.  svpf_analyse_pair for callers that cannot
  * allocate device memory (serving loops, harnesses that forbid hipMalloc): the same
  * arguments and payload, with the search's workspace, sort storage and descriptors

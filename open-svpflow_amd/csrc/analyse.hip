@@ -1218,9 +1218,10 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
 }
 
 // exact_search_level (:480-681) in the reference's visit order: one wavefront per
-// chain.  Used where the parallel sweeps do not apply (the coarse-level bad-block
-// pass keeps a running count over the visit order) and as their fallback.
-__global__ void __launch_bounds__(kWave, 3) search_level_kernel(const Chain* __restrict__ chains) {
+// chain walks the whole level, keeping the bad-block count in a register.  The
+// fallback for a level whose parallel sweeps have not converged within the sweep
+// limit (svpf_analyse_set_max_sweeps); it runs the general search (2 waves per SIMD).
+__global__ void __launch_bounds__(kWave, 2) search_level_kernel(const Chain* __restrict__ chains) {
     const Chain ch = uniform_load(&chains[blockIdx.x]);
     const LevelGeom& g = ch.geom;
     Blk b;

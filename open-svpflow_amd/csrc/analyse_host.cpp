@@ -706,7 +706,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
             st.memo = (Memo*)(W + memo_off[l]);
             st.fseq = (uint32_t*)(W + fseq_off[l]);
             st.seeds = (SeedRec*)(W + seed_off[l]);
-            st.max_sweeps = kMaxSweeps;
+            st.max_sweeps = ctx->max_sweeps > 0 ? ctx->max_sweeps : kMaxSweeps;
             SVPF_HIP(launch_sweep_init(dch, nch_step[l], level_max_n[l], st, ctx->stream));
             int rc2 = seq_step[l] ? SVPF_OK
                                   : run_sweeps(ctx, dch, nch_step[l], level_max_n[l], level_blocks[l], st, bad_step[l],
@@ -872,6 +872,13 @@ int svpf_analyse_batch(svpf_ctx* ctx, const svpf_analyse_params* p, int32_t n, c
 int svpf_analyse_pair(svpf_ctx* ctx, const svpf_analyse_params* p, const svpf_frame* d_cur, const svpf_frame* d_ref,
                       uint8_t* d_payload) {
     return svpf_analyse_batch(ctx, p, 1, d_cur, d_ref, &d_payload);
+}
+
+int svpf_analyse_set_max_sweeps(svpf_ctx* ctx, int32_t max_sweeps) {
+    if (!ctx || max_sweeps < 0 || max_sweeps > kMaxSweeps)
+        return fail(SVPF_ERR_ARG, "svpf_analyse_set_max_sweeps: bad argument");
+    ctx->max_sweeps = max_sweeps;
+    return SVPF_OK;
 }
 
 }  // extern "C"

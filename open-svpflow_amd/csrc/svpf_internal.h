@@ -26,6 +26,8 @@ struct svpf_ctx {
     size_t sort_tmp_cap = 0;
     uint32_t* h_count = nullptr;  // pinned words for reading back sweep queue lengths
     size_t h_count_cap = 0;
+    // Analyse: parallel sweeps per level before the sequential fallback (0 = the default).
+    int max_sweeps = 0;
     // Analyse sweep statistics per level since profiling was last enabled (device, 16 x 4,
     // added by a kernel so that profiling adds no host round trip):
     // {level runs, sweeps launched, evaluations skipped, evaluations run}

@@ -70,6 +70,7 @@ def lib() -> C.CDLL:
         L.svpf_profile_enable.argtypes = [C.c_void_p, C.c_int]
         L.svpf_profile_read.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
         L.svpf_analyse_stats.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64)]
+        L.svpf_analyse_set_max_sweeps.argtypes = [C.c_void_p, C.c_int32]
         for bind in _BINDERS:
             bind(L)
         _lib = L
@@ -162,6 +163,10 @@ class Context:
 
     def profile(self, on: bool) -> None:
         check(lib().svpf_profile_enable(self.h, 1 if on else 0))
+
+    def set_max_sweeps(self, n: int) -> None:
+        """Parallel sweeps per Analyse level before the sequential fallback (0 = default)."""
+        check(lib().svpf_analyse_set_max_sweeps(self.h, n))
 
     def analyse_stats(self, level: int):
         """{runs, sweeps launched, evaluations skipped (inputs unchanged), evaluations run} of one level."""

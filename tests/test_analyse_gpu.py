@@ -127,3 +127,24 @@ def test_analyse_batched_padded(gpu_ctx, oracle):
     print("analyse parity: batch of 4 pairs, padded stride", flush=True)
     frames = synth.clip(352, 200, 5, seed=91, vmax=6.0, nobj=(2, 2), rot_deg=0.0)
     _check(gpu_ctx, oracle, frames, "{pel:2}", CFG2, pad=24, batch=True)
+
+
+@pytest.mark.parametrize("max_sweeps", [8, 24])
+def test_analyse_sequential_fallback(gpu_ctx, oracle, max_sweeps):
+    """A level whose parallel sweeps do not converge within the limit is searched again by
+    the sequential kernel (one wavefront walking the visit order); the payload is the same."""
+    print(f"analyse parity: sequential fallback after {max_sweeps} sweeps", flush=True)
+    frames = synth.clip(320, 192, 3, seed=81, vmax=8.0, nobj=(2, 3), rot_deg=1.0)
+    aopt = "{block:{w:8,h:8,overlap:3},main:{levels:4,search:{type:3,distance:16,satd:true,coarse:{bad:{sad:300}}}}}"
+    gpu_ctx.profile(True)
+    gpu_ctx.set_max_sweeps(max_sweeps)
+    try:
+        _check(gpu_ctx, oracle, frames, "{pel:2}", aopt, batch=True)
+        # sweep statistics are recorded only for levels that converged: level 0 of a
+        # 52x31 grid needs far more than 24 sweeps, so it fell back
+        assert gpu_ctx.analyse_stats(0)["runs"] == 0
+    finally:
+        gpu_ctx.set_max_sweeps(0)
+        gpu_ctx.profile(False)
+    with pytest.raises(svpf.SvpfError):
+        gpu_ctx.set_max_sweeps(5000)
