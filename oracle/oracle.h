@@ -9,17 +9,25 @@
 //
 // Build: oracle/Makefile (g++ -O3 -ffp-contract=off) -> oracle/_build/liboracle.so
 //
-// Pinning.  The reference holds no golden vectors for this path and cannot be
-// built here, so each part is pinned by checks of its own in tests/ (closed forms,
-// textbook restatements, brute force, invariants): the option parser and defaults,
-// the Super build, the SAD/SATD costs, the exhaustive search, the render of
-// identical sources.  These parts have no such check and rest on transcription
-// review alone -- parity unpinned:
-//   - the visit orders, seeds, predictor penalties, bad-block count, global
-//     predictor, interpolation/projection, adapt_lambda, rescale_scores and the
-//     refine pass (exact_recalculate) of analyse.cpp;
-//   - the coverage splat and TruncDiv, the scene-class thresholds and the
-//     magnitude power for exponents other than 1 of render.cpp / smooth.cpp.
+// Pinning.  The reference holds no golden vectors for this path and cannot be built
+// here, so each part is pinned by checks of its own in tests/ that restate the
+// reference's rules independently (closed forms, textbook restatements, brute force,
+// invariants):
+//   - option parser, Super/Analyse/SmoothFps defaults: test_super_oracle.py,
+//     test_lib_exports.py;
+//   - Super build: constant/impulse closed forms and a naive per-pixel restatement;
+//   - block costs: textbook SAD and Hadamard SATD (test_oracle_pins.py);
+//   - the search hierarchy, block by block from the oracle's trace (BlockTrace):
+//     visit orders, predictors, projection, global predictor, lambdas, seeds, the
+//     predictor stage, exhaustive refinement (brute force), the bad-block count,
+//     luma DC, rescale_scores and the refine pass (test_oracle_search_rules.py);
+//   - hex2/UMH/exhaustive over whole frames: pure integer translation gives the exact
+//     shift and fwd = -bwd (test_oracle_pins.py);
+//   - TruncDiv, global predictor, rescale_scores, lambdas, luma_byte, scene classes,
+//     coverage and magnitude masks as closed forms (test_oracle_closed_forms.py);
+//   - the render: identical sources render the source for every algorithm.
+// One rule of the reference has no observable effect and so no check: the lsad/3 of
+// the coarsest level, where the lambda it scales is always 0.
 #pragma once
 #include <cstdint>
 #include <cstddef>

@@ -148,3 +148,32 @@ def test_render_of_static_scene_is_the_source(oracle, algo):
     clip = oracle.SmoothClip(ropt.replace(",limits:{zero:0}", ""), frames, sups, pays, so["sdata"], ao["vdata"],
                              fps=(24, 1))
     assert all(clip.render(k)[1] in (0, 1) for k in range(nout))
+
+
+@pytest.mark.parametrize("search", ["type:3,distance:16,satd:true", "type:2,distance:8"])
+@pytest.mark.parametrize("pel", [1, 2])
+def test_hierarchy_finds_integer_translation(oracle, search, pel):
+    """The whole hierarchy with the default penalties (lambda 10, pnew 50, pzero 100, ...):
+    a frame that is a pure integer translation of the other is matched by pel * shift at
+    every block away from the border, at cost 0, and the two directions are negatives."""
+    w, h, sx, sy = 192, 128, 6, -4
+    big = _frames(31 + pel, w + 32, h + 32)
+    a = tuple(p[8:8 + (h if i == 0 else h // 2), 8:8 + (w if i == 0 else w // 2)] for i, p in enumerate(big))
+    b = tuple(p[8 + (sy if i == 0 else sy // 2):8 + (sy if i == 0 else sy // 2) + (h if i == 0 else h // 2),
+                8 + (sx if i == 0 else sx // 2):8 + (sx if i == 0 else sx // 2) + (w if i == 0 else w // 2)]
+              for i, p in enumerate(big))
+    sopt = "{pel:%d}" % pel
+    so, _ = oracle.super_opts(sopt, w, h)
+    sa, sb = oracle.super_build(sopt, a), oracle.super_build(sopt, b)
+    aopt = "{block:{w:8,h:8,overlap:2},main:{search:{%s}}}" % search
+    hdr, regions = oracle.decode_payload(oracle.analyse(aopt, so["sdata"], w, h, sa, sb))
+    gw, gh = hdr[12], hdr[13]
+    vecs = [r.reshape(gh, gw, -1) for r in regions]
+    m = 2                                          # blocks whose shifted block leaves the frame
+    inner = (slice(m, gh - m), slice(m, gw - m))
+    s0 = 1 if vecs[0][inner][..., 0].flat[0] == pel * sx else -1
+    for r, sign in ((vecs[0], s0), (vecs[1], -s0)):
+        blk = r[inner]
+        assert np.all(blk[..., 0] == sign * pel * sx), np.unique(blk[..., 0], return_counts=True)
+        assert np.all(blk[..., 1] == sign * pel * sy), np.unique(blk[..., 1], return_counts=True)
+        assert np.all(blk[..., 2] == 0)
