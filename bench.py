@@ -1,20 +1,28 @@
 #!/usr/bin/env python3
 """Benchmark of the open-svpflow motion path on one MI355X.
 
-Workload (1080p): a 120-frame 1920x1080 YUV420P8 layered synthetic clip (blurred-noise
-pan up to 12 px/frame, 6-12 rotating textured rectangles, a scene cut at frame 60),
-Super {pel:2} -> Analyse 8x8 blocks, overlap 4, UMH range 16, SATD + chroma, 4 levels
--> SmoothFps algorithm 23 with coverage and magnitude masks at 24->60 fps (298 output
-frames, 238 interpolated).
+Workload (1080p, config 2): a 120-frame 1920x1080 YUV420P8 layered synthetic clip
+(blurred-noise pan up to 12 px/frame, 6-12 rotating textured rectangles, a scene cut at
+frame 60), Super {pel:2} -> Analyse 8x8 blocks, overlap 4, UMH range 16, SATD + chroma,
+4 levels -> SmoothFps algorithm 23 with coverage and magnitude masks at 24->60 fps (298
+output frames, 238 interpolated).
 
 One step = the batched path over the whole clip with the source frames already in
 HBM: one Super launch set for all frames, one Analyse batch for all 120 frame pairs,
 one SmoothFps render batch for all output frames.  `value` is output frames per second
 of that step.  Beside it, measured in the same run:
   * plain: the per-frame path (one Super frame, one Analyse pair, one output frame per
-    call, as a VapourSynth host pulls them) on a mini-clip of the first frames;
-  * roofline of the dominant kernel (the Analyse block search), with HIP events around
-    each of its launches on the library's stream;
+    call, as a VapourSynth host pulls them) over the first 24 source frames, with its
+    SmoothFps session created before the clock starts;
+  * roofline of the dominant kernel set, the Analyse level search, in its own unit:
+    candidate-pixel integer ops (the oracle's candidate counter over the workload's
+    pairs, 7 ops per pixel for SATD) per second against the VALU lane-op peak, timed
+    with HIP events around each level's launches on the library's stream; `traffic` is
+    its measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-
+    calibrated) from profiles/search_traffic.json;
+  * config4: 64 clips of 480p/720p/1080p/1440p with 16-96 frames each, every pair of
+    every clip through svpf_analyse_multi in chunks, every clip rendered (algorithm 23);
+  * serving (config 5): p50/p99 output-frame latency under Poisson arrivals;
   * cpu_baseline: the C++ oracle restatement on the host cores (frame pairs in
     parallel threads, fwd/bwd in 2 threads each), on a mini-clip of the same content.
 """
@@ -40,6 +48,15 @@ ANALYSE = "{block:{w:8,h:8,overlap:3},main:{levels:4,search:{type:3,distance:16,
 SMOOTH = "{rate:{num:5,den:2},algo:23,mask:{area:100},scene:{mode:0}}"
 FPS = (24, 1)
 HBM_PEAK_GBS = 8000.0
+# VALU peak in 32-bit lane operations: 256 CUs x 64 lanes x 2.4 GHz.
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+# The oracle's candidate counter over the 120 pairs of this workload, both directions
+# (python bench.py --count-candidates): pixels compared by every candidate cost the
+# reference evaluates (luma + chroma), per frame pair.  Every level uses SATD here, at
+# 7 integer ops per pixel (16 subtractions, 64 butterfly adds, 16 absolute values and
+# 16 adds per 4x4 tile).
+SEARCH_CANDIDATE_PIXELS_PER_PAIR = 1.4017490e9  # levels 0-3: 1.133e9, 2.192e8, 4.536e7, 4.437e6
+SATD_OPS_PER_PIXEL = 7
 
 
 class Clip:
@@ -72,25 +89,29 @@ class Clip:
         self.smooth.set_sources(0, self.src, self.sup, self.pay)
         self.smooth.render(0, self.out)
 
-    def plain_pass(self, frames: int):
-        """Per-frame calls over the mini-clip of the first `frames` source frames."""
-        sm = svpf.Smooth(self.ctx, SMOOTH, self.w, self.h, FPS[0], FPS[1], frames, self.sdata, self.vdata)
-        for i in range(frames):
+    def plain_session(self, frames: int):
+        """The per-frame path's SmoothFps session over source frames 0 .. frames-1."""
+        return svpf.Smooth(self.ctx, SMOOTH, self.w, self.h, FPS[0], FPS[1], frames, self.sdata, self.vdata)
+
+    def plain_pass(self, sm, frames: int):
+        """Per-frame calls over source frames 0 .. frames-1: a Super frame per call
+        (frames + 1 of them: the last pair's reference), an Analyse pair per call (each
+        against the next source frame, no clamped self-pair), an output frame per call."""
+        for i in range(frames + 1):
             svpf.super_build(self.ctx, self.sp, self.src[i], self.sup[i])
         for i in range(frames):
-            svpf.analyse_pair(self.ctx, self.ap, self.sup[i], self.sup[min(i + 1, frames - 1)], self.pay[i])
+            svpf.analyse_pair(self.ctx, self.ap, self.sup[i], self.sup[i + 1], self.pay[i])
         sm.set_sources(0, self.src[:frames], self.sup[:frames], self.pay[:frames])
         for k in range(sm.output_frames):
             sm.render(k, self.out[k:k + 1])
-        nout = sm.output_frames
         self.ctx.sync()
-        sm.close()
-        return nout
+        return sm.output_frames
 
 
 def search_bytes_per_step(clip: Clip) -> float:
     """Algorithmic bytes of the Analyse search: every level plane of the current and the
-    reference super frame read once per frame pair (pel^2 subplanes of luma + 2 chroma)."""
+    reference super frame read once per frame pair (luma + 2 chroma; the pel^2 subplanes
+    exist at level 0 only)."""
     pel = clip.sp.pel
     total = 0.0
     w, h = clip.w, clip.h
@@ -98,7 +119,7 @@ def search_bytes_per_step(clip: Clip) -> float:
         lw, lh = w, h
         for _ in range(lv):
             lw, lh = 2 * (lw // 4), 2 * (lh // 4)
-        total += pel * pel * (lw * lh + 2 * (lw // 2) * (lh // 2))
+        total += (pel * pel if lv == 0 else 1) * (lw * lh + 2 * (lw // 2) * (lh // 2))
     return 2.0 * total * clip.n
 
 
@@ -109,6 +130,25 @@ def render_bytes_per_step(clip: Clip, rendered: int, copies: int) -> float:
     gw, gh = clip.vdata[11], clip.vdata[12]
     grid = (gw + 1) * (gh + 1)
     return rendered * (3 * fb + 2 * 8 * grid + 3 * grid) + copies * 2 * fb
+
+
+def count_candidates(frames, threads: int = 8):
+    """The oracle's candidate counters over every frame pair of the workload: per level,
+    the candidate evaluations and the pixels they compare (luma + chroma), both
+    directions.  Host only (no GPU); it gives SEARCH_OPS_PER_PAIR below."""
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    import oracle_lib as O
+    from concurrent.futures import ThreadPoolExecutor
+    h, w = frames[0][0].shape
+    so, _ = O.super_opts(SUPER, w, h)
+    with ThreadPoolExecutor(threads) as ex:
+        sups = list(ex.map(lambda f: O.super_build(SUPER, f), frames))
+        n = len(frames)
+        res = list(ex.map(lambda i: O.analyse(ANALYSE, so["sdata"], w, h, sups[i], sups[min(i + 1, n - 1)],
+                                              counters=True)[1:], range(n)))
+    evals = [sum(r[0][lv] for r in res) for lv in range(16)]
+    pixels = [sum(r[1][lv] for r in res) for lv in range(16)]
+    return evals, pixels
 
 
 def cpu_baseline(frames, pairs: int):
@@ -151,6 +191,84 @@ def cpu_baseline(frames, pairs: int):
     [t.join() for t in th]
     dt = time.perf_counter() - t0
     return nout, dt, 2 * len(sub)
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+# ------------------------------------------------------------------------- config 4
+
+C4_RES = ((854, 480), (1280, 720), (1920, 1080), (2560, 1440))
+C4_ANALYSE = "{block:{w:8,h:8,overlap:3},main:{search:{type:3,distance:16,satd:true}}}"  # levels auto
+
+
+class C4Clip:
+    """One clip of the batched workload: its source frames (from a device pool of the
+    layered generator, played back and forth from a per-clip offset), and its own super
+    frames, vector payloads, SmoothFps session and output frames."""
+
+    def __init__(self, ctx, w, h, n, pool, offset):
+        self.w, self.h, self.n = w, h, n
+        order = list(range(len(pool))) + list(range(len(pool) - 2, 0, -1))
+        self.src = [pool[order[(offset + i) % len(order)]] for i in range(n)]
+        self.sp, self.sdata = svpf.super_params(SUPER, w, h)
+        sh = svpf.super_height(self.sp)
+        strides = (w, w // 2, w // 2)
+        self.sup = [svpf.DeviceFrame.alloc(ctx, strides, (sh, sh // 2, sh // 2)) for _ in range(n)]
+        self.ap, self.vdata, self.plen = svpf.analyse_params(C4_ANALYSE, self.sdata, w, h)
+        self.pay = [ctx.alloc(self.plen) for _ in range(n)]
+        self.refs = [self.sup[min(i + 1, n - 1)] for i in range(n)]
+        self.sm = svpf.Smooth(ctx, SMOOTH, w, h, FPS[0], FPS[1], n, self.sdata, self.vdata)
+        self.nout = self.sm.output_frames
+        self.out = [svpf.DeviceFrame.alloc(ctx, strides, (h, h // 2, h // 2)) for _ in range(self.nout)]
+
+
+def config4_setup(ctx, clips: int = 64, seed: int = 4000, pool_frames: int = 10):
+    rng = np.random.default_rng(seed)
+    pools = {}
+    for i, (w, h) in enumerate(C4_RES):  # two seeds (content and velocities) per size
+        for k in range(2):
+            frames = synth.clip(w, h, pool_frames, seed=seed + 10 * i + k, vmax=16.0, nobj=(6, 12), rot_deg=2.0)
+            dev = []
+            for f in frames:
+                d = svpf.DeviceFrame.alloc(ctx, (w, w // 2, w // 2), (h, h // 2, h // 2))
+                d.upload(f)
+                dev.append(d)
+            pools[(w, h, k)] = dev
+    out = []
+    for c in range(clips):
+        w, h = C4_RES[int(rng.integers(len(C4_RES)))]
+        n = int(rng.integers(16, 97))
+        out.append(C4Clip(ctx, w, h, n, pools[(w, h, int(rng.integers(2)))], int(rng.integers(2 * pool_frames))))
+    ctx.sync()
+    return out
+
+
+def config4_pass(ctx, clips, pairs_per_call: int):
+    """Super for every frame (one batched call per clip), Analyse for every pair of every
+    clip (svpf_analyse_multi over chunks of whole clips, mixing sizes and pyramid
+    depths), then each clip's SmoothFps render of all its output frames."""
+    for c in clips:
+        svpf.super_build_batch(ctx, c.sp, c.src, c.sup)
+    chunk = []
+    for c in clips + [None]:
+        if c is None or (chunk and sum(x.n for x in chunk) + c.n > pairs_per_call):
+            svpf.analyse_multi(ctx, [x.ap for x in chunk for _ in range(x.n)], [f for x in chunk for f in x.sup],
+                               [f for x in chunk for f in x.refs], [p for x in chunk for p in x.pay])
+            chunk = []
+        if c is not None:
+            chunk.append(c)
+    for c in clips:
+        c.sm.set_sources(0, c.src, c.sup, c.pay)
+        c.sm.render(0, c.out)
 
 
 SERVE_RES = ((854, 480), (1280, 720), (1920, 1080), (2560, 1440))
@@ -200,20 +318,42 @@ def serving_run(ctx, seconds: float, pixel_rate_budget: float, seed: int = 5000,
     return stats
 
 
+def search_traffic():
+    """Measured HBM bytes of the Analyse search per step (profiles/search_traffic.json,
+    written from two rocprofv3 counter passes over this workload), or None."""
+    path = os.path.join(HERE, "profiles", "search_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--frames", type=int, default=120)
-    ap.add_argument("--plain-frames", type=int, default=3, help="mini-clip length of the per-frame path sample")
+    ap.add_argument("--plain-frames", type=int, default=24, help="source frames of the per-frame path sample")
     ap.add_argument("--cpu-pairs", type=int, default=8, help="frame pairs of the CPU baseline sample (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=15.0,
                     help="length of the config-5 serving run that gives the p50/p99 frame latency (0: skip)")
+    ap.add_argument("--config4-clips", type=int, default=64, help="clips of the config-4 batched run (0: skip)")
+    ap.add_argument("--config4-pairs-per-call", type=int, default=480,
+                    help="frame pairs per svpf_analyse_multi call of the config-4 run (whole clips)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run 4 device-to-device copies of MIB MiB each (the known-size reference for "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE scaling)")
+    ap.add_argument("--count-candidates", action="store_true",
+                    help="host only: print the oracle's candidate counters over the workload's frame pairs and exit")
     args = ap.parse_args()
+    if args.count_candidates:
+        frames = synth.config_clip(2, frames=args.frames)
+        evals, pixels = count_candidates(frames)
+        print(json.dumps({"pairs": len(frames), "evaluations_per_level": evals[:8], "pixels_per_level": pixels[:8],
+                          "candidate_pixels_per_pair": sum(pixels) / len(frames)}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the motion path shards by clip / frame range with no collective "
               "(see the sharding notes); run it with --gpus 1", file=sys.stderr)
@@ -257,18 +397,33 @@ def main():
     ms_per_step = 1000.0 * wall / args.steps
     value = clip.nout * args.steps / wall
 
-    # plain per-frame path on a mini-clip
-    plain_out, plain_dt, plain_fps = 0, 0.0, 0.0
+    # plain per-frame path: the first plain_frames source frames, session made beforehand
+    plain = None
     if args.plain_frames > 1:
-        print(f"[bench] plain per-frame path, {args.plain_frames} frames", file=sys.stderr, flush=True)
+        nf = min(args.plain_frames, clip.n - 1)
+        print(f"[bench] plain per-frame path, {nf} source frames", file=sys.stderr, flush=True)
+        sm = clip.plain_session(nf)
+        clip.plain_pass(sm, nf)  # one untimed pass: the per-pair workspace and descriptors exist
+        ctx.sync()
         t0 = time.perf_counter()
-        plain_out = clip.plain_pass(args.plain_frames)
+        plain_out = clip.plain_pass(sm, nf)
         plain_dt = time.perf_counter() - t0
-        plain_fps = plain_out / plain_dt
+        sm.close()
+        plain = {"value": round(plain_out / plain_dt, 3), "unit": "fps", "source_frames": nf,
+                 "output_frames": plain_out, "seconds": round(plain_dt, 3),
+                 "ms_per_output_frame": round(1000.0 * plain_dt / plain_out, 3),
+                 "ms_per_source_frame": round(1000.0 * plain_dt / nf, 3)}
 
+    # roofline of the Analyse search: candidate-pixel integer ops against the VALU peak
     search_ms, search_n = prof["search"]
+    pixels_per_pair = SEARCH_CANDIDATE_PIXELS_PER_PAIR
+    ops = pixels_per_pair * SATD_OPS_PER_PIXEL * clip.n * args.steps
+    search_tops = ops / (search_ms / 1000.0) / 1e12 if search_ms > 0 else 0.0
     sb = search_bytes_per_step(clip) * args.steps
-    search_gbs = sb / (search_ms / 1000.0) / 1e9 if search_ms > 0 else 0.0
+    tr = search_traffic()
+    traffic = None
+    if tr:
+        traffic = (tr["fetch_bytes_per_step"] + tr["write_bytes_per_step"]) / tr["level_launches_per_step"]
     render_ms, render_n = prof["render"]
     rb = render_bytes_per_step(clip, rendered, copies) * args.steps
     render_gbs = rb / (render_ms / 1000.0) / 1e9 if render_ms > 0 else 0.0
@@ -276,6 +431,34 @@ def main():
     fb = clip.w * clip.h * 1.5
     sbytes = (fb + clip.super_rows * clip.w * 1.5) * clip.n * args.steps
     super_gbs = sbytes / (super_ms / 1000.0) / 1e9 if super_ms > 0 else 0.0
+
+    c4 = None
+    if args.config4_clips > 0:
+        print(f"[bench] config 4: {args.config4_clips} clips", file=sys.stderr, flush=True)
+        clips = config4_setup(ctx, args.config4_clips)
+        config4_pass(ctx, clips, args.config4_pairs_per_call)  # warm-up (workspace, sessions' prep buffers)
+        ctx.sync()
+        t0 = time.perf_counter()
+        config4_pass(ctx, clips, args.config4_pairs_per_call)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        nout = sum(c.nout for c in clips)
+        by_res = {f"{w}x{h}": sum(1 for c in clips if (c.w, c.h) == (w, h)) for (w, h) in C4_RES}
+        c4 = {"value": round(nout / dt, 3), "unit": "fps", "seconds": round(dt, 3), "clips": len(clips),
+              "source_frames": sum(c.n for c in clips), "output_frames": nout, "clips_by_size": by_res,
+              "pairs_per_analyse_call": args.config4_pairs_per_call,
+              "source_megapixels_per_s": round(sum(c.n * c.w * c.h for c in clips) / dt / 1e6, 1)}
+        for c in clips:
+            c.sm.close()
+            for f in c.sup + c.out:
+                for ptr in (f.y, f.u, f.v):
+                    ctx.free(ptr)
+            for ptr in c.pay:
+                ctx.free(ptr)
+        for dev in {id(f): f for c in clips for f in c.src}.values():
+            for ptr in (dev.y, dev.u, dev.v):
+                ctx.free(ptr)
+        del clips
 
     serving = None
     if args.serve_seconds > 0:
@@ -289,11 +472,12 @@ def main():
     if args.cpu_pairs > 0:
         print(f"[bench] cpu baseline, {args.cpu_pairs} pairs", file=sys.stderr, flush=True)
         nout, dt, threads = cpu_baseline(frames, args.cpu_pairs)
-        cpu = {"value": round(nout / dt, 3), "unit": "fps", "cores": threads, "kind": "port",
-               "sample": f"oracle (C++ restatement) on a {args.cpu_pairs + 1}-frame 1080p mini-clip of the same "
-                         f"content: Super, Analyse ({args.cpu_pairs + 1} pairs in parallel threads, fwd/bwd in 2 "
-                         f"threads each), SmoothFps of all {nout} output frames; {dt:.1f} s wall, "
-                         f"os.cpu_count()={os.cpu_count()}"}
+        cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        cpu = {"value": round(nout / dt, 3), "unit": "fps", "cores": min(cores, threads), "threads": threads,
+               "cores_available": cores, "cpu_model": cpu_model(), "kind": "port",
+               "sample": f"oracle (C++ restatement, g++ -O3) on a {args.cpu_pairs + 1}-frame 1080p mini-clip of "
+                         f"the same content: Super, Analyse ({args.cpu_pairs + 1} pairs in parallel threads, fwd/bwd "
+                         f"in 2 threads each), SmoothFps of all {nout} output frames; {dt:.1f} s wall"}
 
     line = {
         "metric": "output fps at 1080p (bit-exact MVs), batched vs plain GPU; p99 frame latency",
@@ -312,15 +496,20 @@ def main():
                                f"({rendered} interpolated): Super {SUPER}, Analyse {ANALYSE}, SmoothFps {SMOOTH} "
                                f"at {FPS[0]}/{FPS[1]} fps",
                    "path": "batched (whole clip per launch set)", "frames": clip.n, "output_frames": clip.nout},
-        "plain": {"value": round(plain_fps, 3), "unit": "fps", "frames": args.plain_frames,
-                  "output_frames": plain_out, "seconds": round(plain_dt, 3)},
-        "batched_over_plain": round(value / plain_fps, 2) if plain_fps > 0 else None,
+        "plain": plain,
+        "batched_over_plain": round(value / plain["value"], 2) if plain else None,
         "gpu_ms_per_step_events": round(gpu_ms / args.steps, 2),
-        "roofline": {"bound": "hbm", "kernel": "Analyse level search (sweep0/deferred/validate/sweep kernels of one "
-                                                "pyramid level; integer-VALU bound)",
-                     "achieved": round(search_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": search_gbs / HBM_PEAK_GBS, "traffic": None,
+        "roofline": {"bound": "valu",
+                     "kernel": "Analyse level search (sweep0/deferred/validate/sweep/scan kernels of one pyramid "
+                               "level, HIP events around each level)",
+                     "achieved": round(search_tops, 4), "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",
+                     "frac": search_tops / VALU_PEAK_TOPS,
+                     "traffic": traffic,
+                     "traffic_source": tr.get("source") if tr else None,
                      "launches": search_n, "avg_launch_ms": round(search_ms / max(search_n, 1), 3),
+                     "algorithmic_ops_per_launch": ops / max(search_n, 1),
+                     "algorithmic_work": f"{pixels_per_pair:.4g} candidate pixels per frame pair (oracle counter, "
+                                         f"both directions, all levels) x {SATD_OPS_PER_PIXEL} ops (SATD)",
                      "algorithmic_bytes_per_launch": sb / max(search_n, 1)},
         "roofline_render": {"bound": "hbm", "achieved": round(render_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": render_gbs / HBM_PEAK_GBS, "launches": render_n,
@@ -330,6 +519,7 @@ def main():
                            "avg_launch_ms": round(super_ms / max(super_n, 1), 3)},
         "stage_ms_per_step": {k: round(v[0] / args.steps, 2) for k, v in prof.items()},
         "analyse_sweeps": {f"level{lv}": v for lv, v in sweeps.items()},
+        "config4": c4,
         "p99_frame_latency_ms": serving["p99_ms"] if serving else None,
         "serving": serving,
         "cpu_baseline": cpu,
