@@ -196,6 +196,16 @@ int svpf_smooth_set_sources(svpf_smooth* s, int32_t first, int32_t count, const 
  * (kind, algo): kind 0 = copy of source n, 1 = copy of n+1, 2 = interpolated with
  * `algo`; the call then synchronizes. */
 int svpf_smooth_render(svpf_smooth* s, int32_t k0, int32_t count, const svpf_frame* d_out, int32_t* h_decision);
+
+/* Routes the session's rendered frames through the reference's float GPU renderer
+ * (the OpenCL kernel of crates/svpflow2/src/gpu.rs:1039-1168, which the reference uses
+ * on a GPU, core.rs:1429, 1630-1653) instead of the bit-exact integer CPU renderer
+ * (renderer.rs), when `on` != 0: B-spline cubic vector sampling, bilinear taps, luma
+ * blended in linear light.  Its results depend on floating-point rounding, so they
+ * agree with the integer renderer only within a tolerance.  Sessions with block
+ * vectors keep the integer renderer, as the reference does; so do sessions with
+ * scene blending (the reference falls back only for their zero-origin frames). */
+int svpf_smooth_set_float_render(svpf_smooth* s, int32_t on);
 int32_t svpf_smooth_output_frames(const svpf_smooth* s);
 /* Output frame rate of the session (FilterState::output_info, core.rs:603-613). */
 int svpf_smooth_output_rate(const svpf_smooth* s, int64_t* fps_num, int64_t* fps_den);

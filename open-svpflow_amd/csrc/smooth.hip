@@ -330,6 +330,7 @@ __device__ __forceinline__ int div12(int v, bool half) {
 // generic, which compiles to flat loads).
 typedef __attribute__((address_space(1))) const uint8_t gm_u8;
 typedef __attribute__((address_space(1))) const uint16_t gm_u16;
+typedef __attribute__((address_space(1))) const uint16_t gm_u16;
 
 __device__ __forceinline__ int mval(const uint16_t* plane, int idx, int count, int t) {  // motion_value + LUT
     const int li = min(idx < count ? (int)((gm_u16*)plane)[idx] : (int)kNeutral, 2047);
@@ -504,6 +505,172 @@ __global__ void __launch_bounds__(256) render_kernel(const FrameJob* __restrict_
 
 }  // namespace
 
+// ------------------------------------------------------------------ float renderer
+// The reference's float GPU renderer (crates/svpflow2/src/gpu.rs:1039-1168, fed as
+// gpu.rs:754-957 and core.rs:3235-3330 feed it): vectors sampled with a B-spline cubic
+// made of four bilinear fetches, source taps bilinear with clamp-to-edge, luma blended in
+// linear light.  Image filtering is computed in fp32 with the weights the OpenCL
+// specification writes (its precision is the device's choice there), division is IEEE.
+// One thread per pixel of one plane of one frame.
+
+struct F4 {
+    float x, y, z, w;
+};
+__device__ __forceinline__ float fmix(float x, float y, float a) { return x + (y - x) * a; }
+__device__ __forceinline__ F4 fmix4(const F4& x, const F4& y, float a) {
+    return F4{fmix(x.x, y.x, a), fmix(x.y, y.y, a), fmix(x.z, y.z, a), fmix(x.w, y.w, a)};
+}
+__device__ __forceinline__ float median3f(float a, float b, float c) {
+    const float lo = fminf(a, b);
+    return fmaxf(lo, fminf(a + b - lo, c));
+}
+
+// read_imagef, CLK_FILTER_LINEAR | CLK_ADDRESS_CLAMP_TO_EDGE, unnormalized (u, v)
+template <class Tex>
+__device__ __forceinline__ float fbilinear(const Tex& tex, int w, int h, float u, float v) {
+    const float x = u - 0.5f, y = v - 0.5f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float a = x - fx, b = y - fy;
+    const int i0 = (int)fx, j0 = (int)fy;
+    const int xa = clampi(i0, 0, w - 1), xb = clampi(i0 + 1, 0, w - 1);
+    const int ya = clampi(j0, 0, h - 1), yb = clampi(j0 + 1, 0, h - 1);
+    const float one_a = 1.0f - a, one_b = 1.0f - b;
+    return one_a * one_b * tex(xa, ya) + a * one_b * tex(xb, ya) + one_a * b * tex(xa, yb) + a * b * tex(xb, yb);
+}
+
+__device__ __forceinline__ void cubic_axis(float p, float& g0, float& h0, float& h1) {
+    const float grid = p - 0.5f;
+    const float index = floorf(grid);
+    const float f = grid - index, r = 1.0f - f;
+    const float w0 = r * r * r / 6.0f;
+    const float w1 = 2.0f / 3.0f - 0.5f * f * f * (2.0f - f);
+    const float w2 = 2.0f / 3.0f - 0.5f * r * r * (2.0f - r);
+    const float w3 = f * f * f / 6.0f;
+    g0 = w0 + w1;
+    const float g1 = w2 + w3;
+    h0 = w1 / g0 - 0.5f + index;
+    h1 = w3 / g1 + 1.5f + index;
+}
+
+// cubic_sample of a 4-channel image given as four planes of the motion grid.
+template <class T, class Conv>
+__device__ __forceinline__ F4 fcubic(const T* c0, const T* c1, const T* c2, const T* c3, int w, int h, float px,
+                                     float py, const Conv& conv) {
+    float gx, h0x, h1x, gy, h0y, h1y;
+    cubic_axis(px, gx, h0x, h1x);
+    cubic_axis(py, gy, h0y, h1y);
+    auto fetch = [&](float u, float v) {
+        return F4{fbilinear([&](int x, int y) { return conv(c0[y * w + x]); }, w, h, u, v),
+                  fbilinear([&](int x, int y) { return conv(c1[y * w + x]); }, w, h, u, v),
+                  fbilinear([&](int x, int y) { return conv(c2[y * w + x]); }, w, h, u, v),
+                  fbilinear([&](int x, int y) { return conv(c3[y * w + x]); }, w, h, u, v)};
+    };
+    const F4 a = fetch(h0x, h0y), b = fetch(h1x, h0y), c = fetch(h0x, h1y), d = fetch(h1x, h1y);
+    return fmix4(fmix4(d, b, gy), fmix4(c, a, gy), gx);
+}
+
+__global__ void __launch_bounds__(256) render_float_kernel(const FrameJob* __restrict__ jobs,
+                                                           const FramePlan* __restrict__ plans,
+                                                           const SmoothGeom* __restrict__ gp) {
+    const int plane = blockIdx.z % 3;
+    const FrameJob& j = jobs[blockIdx.z / 3];
+    const SmoothGeom& g = *gp;
+    const bool chroma = plane != 0;
+    const int W = chroma ? g.width / 2 : g.width, H = chroma ? g.height / 2 : g.height;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    const FramePlan plan = plans[blockIdx.z / 3];
+    uint8_t* out = j.dst[plane] + (size_t)y * j.dstride[plane] + x;
+    if (plan.decision != 2) {  // copies, as the integer path
+        const uint8_t* s = plan.decision == 1 ? j.src1[plane] : j.src0[plane];
+        *out = s[(size_t)y * j.sstride[plane] + x];
+        return;
+    }
+    const int mw = g.mw, mh = g.mh;
+    const int algo = (plan.algo == 23 && !plan.mode23_ready) ? 21 : plan.algo;
+    const bool has_sad = g.mask_area != 0;
+    const int xr = chroma ? 2 : 1, yr = chroma ? 2 : 1;
+    const gm_u8* s0 = (const gm_u8*)j.src0[plane];
+    const gm_u8* s1 = (const gm_u8*)j.src1[plane];
+    const int ss = j.sstride[plane];
+    auto tex0 = [&](int tx, int ty) {
+        const uint8_t t = s0[ty * ss + tx];
+        return chroma ? (float)t / 255.0f : g.lin_lut[t];
+    };
+    auto tex1 = [&](int tx, int ty) {
+        const uint8_t t = s1[ty * ss + tx];
+        return chroma ? (float)t / 255.0f : g.lin_lut[t];
+    };
+    const float maxx = (float)(W - 1), maxy = (float)(H - 1);
+    const float denx = (float)xr * 256.0f, deny = (float)yr * 256.0f;  // pel = source_step = 1
+    auto source = [&](bool second, float vx, float vy, int time) {
+        float dx = vx * 65535.0f - 1024.0f, dy = vy * 65535.0f - 1024.0f;
+        dx = dx * (float)time / denx;
+        dy = dy * (float)time / deny;
+        const float px = fminf(fmaxf((float)x + dx, 0.0f), maxx);
+        const float py = fminf(fmaxf((float)y + dy, 0.0f), maxy);
+        return 255.0f * (second ? fbilinear(tex1, W, H, px + 0.5f, py + 0.5f)
+                                : fbilinear(tex0, W, H, px + 0.5f, py + 0.5f));
+    };
+    auto u16n = [](uint16_t v) { return (float)v / 65535.0f; };
+    auto u8n = [](uint8_t v) { return (float)v / 255.0f; };
+    const float vpx = (float)(x * xr - g.cox) / (float)g.cbw;
+    const float vpy = (float)(y * yr - g.coy) / (float)g.cbh;
+    const gm_u16* m0x = (const gm_u16*)j.mv_n[0];
+    const gm_u16* m0y = (const gm_u16*)j.mv_n[1];
+    const gm_u16* m1x = (const gm_u16*)j.mv_n[2];
+    const gm_u16* m1y = (const gm_u16*)j.mv_n[3];
+    const F4 v = fcubic(m0x, m1y, m1x, m0y, mw, mh, vpx, vpy, u16n);
+    const float ref_f = source(false, v.x, v.w, plan.phase);
+    const float ref_b = source(true, v.z, v.y, 256 - plan.phase);
+    const float ref_f0 = 255.0f * fbilinear(tex0, W, H, (float)x + 0.5f, (float)y + 0.5f);
+    const float ref_b0 = 255.0f * fbilinear(tex1, W, H, (float)x + 0.5f, (float)y + 0.5f);
+    const float time = (float)plan.phase / 256.0f;
+    F4 mask{0.0f, 0.0f, 0.0f, 0.0f};
+    if (algo >= 21 || has_sad) {
+        // (area b, coverage a, coverage b, area a); a plane the frame lacks reads as 0
+        const gm_u8* c0 = (const gm_u8*)(has_sad ? j.mag[1] : j.cov[0]);
+        const gm_u8* c1 = (const gm_u8*)(algo >= 21 ? j.cov[0] : j.mag[0]);
+        const gm_u8* c2 = (const gm_u8*)(algo >= 21 ? j.cov[1] : j.mag[0]);
+        const gm_u8* c3 = (const gm_u8*)(has_sad ? j.mag[0] : j.cov[0]);
+        mask = fcubic(c0, c1, c2, c3, mw, mh, vpx, vpy, u8n);
+        if (!has_sad) { mask.x = 0.0f; mask.w = 0.0f; }
+        if (algo < 21) { mask.y = 0.0f; mask.z = 0.0f; }
+    }
+    float result;
+    if (algo == 1) {
+        result = ref_b;
+    } else if (algo == 2) {
+        result = ref_f;
+    } else if (algo == 11) {
+        result = fmix(ref_f, ref_b, time);
+    } else if (algo == 13) {
+        result = median3f(ref_f, ref_b, fmix(ref_f0, ref_b0, time));
+    } else if (algo == 21) {
+        result = fmix(fmix(ref_f, ref_b, mask.y), fmix(ref_b, ref_f, mask.z), time);
+    } else if (algo == 22) {
+        result = median3f(fmix(ref_f, ref_b, mask.y), fmix(ref_b, ref_f, mask.z), fmix(ref_f0, ref_b0, time));
+    } else {
+        const F4 e = fcubic((const gm_u16*)j.mv2[0], (const gm_u16*)j.mv3[1], (const gm_u16*)j.mv3[0],
+                            (const gm_u16*)j.mv2[1], mw, mh, vpx, vpy, u16n);
+        const float ref_ff = source(false, e.x, e.w, plan.phase);
+        const float ref_bb = source(true, e.z, e.y, 256 - plan.phase);
+        result = fmix(fmix(ref_f, median3f(ref_b, ref_bb, ref_f), mask.y),
+                      fmix(ref_b, median3f(ref_b, ref_ff, ref_f), mask.z), time);
+    }
+    if (has_sad) {
+        if (algo == 1) result = fmix(result, ref_b0, mask.x);
+        else if (algo == 2) result = fmix(result, ref_f0, mask.w);
+        else result = fmix(result, fmix(ref_f0, ref_b0, time), fmix(mask.x, mask.w, time));
+    }
+    if (!chroma) {  // linear light back to the gamma curve
+        float val = result / 255.0f;
+        val = val < 0.018f ? val * 4.5f : 1.099f * powf(val, 0.45f) - 0.099f;
+        result = val * 255.0f;
+    }
+    *out = (uint8_t)fminf(fmaxf(roundf(result), 0.0f), 255.0f);
+}
+
 hipError_t launch_smooth_prep(const PrepJob* d_jobs, int njobs, const SmoothGeom* d_geom, int count, int mc,
                               hipStream_t s) {
     dim3 g1((count + 255) / 256, njobs), g2((mc + 255) / 256, njobs);
@@ -530,6 +697,13 @@ hipError_t launch_smooth_render(const FrameJob* d_jobs, const FramePlan* d_plans
                                 int width, int height, hipStream_t s) {
     dim3 grid((width + 255) / 256, height, 3 * njobs);
     hipLaunchKernelGGL(render_kernel, grid, dim3(256), 0, s, d_jobs, d_plans, d_geom);
+    return hipGetLastError();
+}
+
+hipError_t launch_smooth_render_float(const FrameJob* d_jobs, const FramePlan* d_plans, int njobs,
+                                      const SmoothGeom* d_geom, int width, int height, hipStream_t s) {
+    dim3 grid((width + 255) / 256, height, 3 * njobs);
+    hipLaunchKernelGGL(render_float_kernel, grid, dim3(256), 0, s, d_jobs, d_plans, d_geom);
     return hipGetLastError();
 }
 

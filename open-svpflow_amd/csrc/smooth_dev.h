@@ -31,6 +31,8 @@ struct SmoothGeom {
     const uint8_t* mag_lut;
     int32_t lim_zero, lim_m1, lim_m2, lim_scene, lim_blocks;
     int32_t nframes;
+    int32_t float_render;            // rendered frames through the float renderer (svpf_smooth_set_float_render)
+    float lin_lut[256];              // linear-light luma of each 8-bit sample (the float renderer's luma image)
     uint8_t luma_lut[512];           // luma_byte((a + b) / denom, gamma) for a + b in 0..510
     uint8_t cover_lut[4097];         // coverage_byte for remaining = 0..area (area <= 4096)
 };

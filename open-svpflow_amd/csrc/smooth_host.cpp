@@ -19,6 +19,8 @@ hipError_t launch_smooth_masks(const FrameJob* d_jobs, FramePlan* d_plans, int n
                                int mw, int mh, bool coverage, hipStream_t s);
 hipError_t launch_smooth_render(const FrameJob* d_jobs, const FramePlan* d_plans, int njobs, const SmoothGeom* d_geom,
                                 int width, int height, hipStream_t s);
+hipError_t launch_smooth_render_float(const FrameJob* d_jobs, const FramePlan* d_plans, int njobs,
+                                      const SmoothGeom* d_geom, int width, int height, hipStream_t s);
 
 namespace {
 
@@ -435,6 +437,10 @@ int svpf_smooth_create(svpf_ctx* ctx, const char* opt, size_t opt_len, int32_t w
         const int sc = trunc_i32(std::pow((double)i / denom, s->opt.luma) * 255.0);
         g.luma_lut[i] = (uint8_t)(std::max(sc, 20) & 0xFF);
     }
+    for (int t = 0; t < 256; t++) {  // linear_luma (gpu.rs): the float renderer's luma image
+        const float v = (float)t / 255.0f;
+        g.lin_lut[t] = v < 0.081f ? v / 4.5f : std::pow((v + 0.099f) / 1.099f, 1.0f / 0.45f);
+    }
     const int area = std::max(vbw * vbh, 1);
     const double strength = (double)g.mask_cover / 100.0;
     for (int r = 0; r <= area && r <= 4096; r++) {  // coverage_byte LUT (renderer.rs:3112-3117, 3262-3272)
@@ -638,7 +644,12 @@ int svpf_smooth_render(svpf_smooth* s, int32_t k0, int32_t count, const svpf_fra
                                  s->ctx->stream));
     prof_end(s->ctx, ph);
     ph = prof_begin(s->ctx, kProfRender);
-    SVPF_HIP(launch_smooth_render((const FrameJob*)d_up, d_plans, count, s->d_geom, s->width, s->height, s->ctx->stream));
+    if (s->geom.float_render)
+        SVPF_HIP(launch_smooth_render_float((const FrameJob*)d_up, d_plans, count, s->d_geom, s->width, s->height,
+                                            s->ctx->stream));
+    else
+        SVPF_HIP(launch_smooth_render((const FrameJob*)d_up, d_plans, count, s->d_geom, s->width, s->height,
+                                      s->ctx->stream));
     prof_end(s->ctx, ph);
     if (h_decision) {  // (decision, algo) per frame; synchronizes
         SVPF_HIP(hipMemcpyAsync(plans.data(), d_plans, (size_t)count * sizeof(FramePlan), hipMemcpyDeviceToHost,
@@ -649,6 +660,18 @@ int svpf_smooth_render(svpf_smooth* s, int32_t k0, int32_t count, const svpf_fra
             h_decision[2 * i + 1] = plans[(size_t)i].algo;
         }
     }
+    return SVPF_OK;
+}
+
+int svpf_smooth_set_float_render(svpf_smooth* s, int32_t on) {
+    if (!s) return fail(SVPF_ERR_ARG, "svpf_smooth_set_float_render: null session");
+    // the reference renders on its GPU only with interpolated vectors and without scene
+    // blending (core.rs:1429); other sessions keep the integer renderer
+    const int32_t v = (on && s->geom.interp && !s->opt.blend) ? 1 : 0;
+    if (v == s->geom.float_render) return SVPF_OK;
+    SVPF_HIP(hipStreamSynchronize(s->ctx->stream));  // queued renders read the geometry
+    s->geom.float_render = v;
+    SVPF_HIP(hipMemcpy(&s->d_geom->float_render, &v, sizeof v, hipMemcpyHostToDevice));
     return SVPF_OK;
 }
 

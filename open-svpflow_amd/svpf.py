@@ -321,6 +321,7 @@ def _bind_smooth(L):
     L.svpf_smooth_output_frames.argtypes = [C.c_void_p]
     L.svpf_smooth_output_frames.restype = C.c_int32
     L.svpf_smooth_release.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    L.svpf_smooth_set_float_render.argtypes = [C.c_void_p, C.c_int32]
 
 
 _BINDERS.append(_bind_smooth)
@@ -368,6 +369,11 @@ class Smooth:
         U = (Frame * n)(*[f.desc() for f in supers]) if supers is not None else None
         V = (C.c_void_p * n)(*d_vectors)
         check(lib().svpf_smooth_set_sources(self.h, first, n, S, U, V))
+
+    def set_float_render(self, on: bool) -> None:
+        """Rendered frames through the reference's float GPU renderer (True) or the
+        bit-exact integer renderer (False, the default)."""
+        check(lib().svpf_smooth_set_float_render(self.h, 1 if on else 0))
 
     def release(self, first: int, count: int) -> None:
         """Frees the device state derived from source frames first..first+count-1."""

@@ -290,7 +290,7 @@ int oracle_smooth_render(const char* opt, size_t len, int width, int height, int
                          const uint8_t* const* src_u, const uint8_t* const* src_v, const uint8_t* const* sup_y,
                          const uint8_t* const* sup_u, const uint8_t* const* sup_v, int super_rows,
                          const uint8_t* const* vec, size_t vec_len, int k, uint8_t* dy, uint8_t* du, uint8_t* dv,
-                         int32_t* algo_out) {
+                         int32_t* algo_out, int float_render) {
     Value v;
     bool have = parse_opt(opt, len, &v);
     SmoothClip c;
@@ -311,6 +311,7 @@ int oracle_smooth_render(const char* opt, size_t len, int width, int height, int
     }
     c.super_rows = super_rows;
     c.vec_len = vec_len;
+    c.float_render = float_render != 0;
     uint8_t* dst[3] = {dy, du, dv};
     int algo = 0;
     int r = smooth_render_frame(c, k, dst, &algo);

@@ -86,6 +86,26 @@ private:
     int16_t inverse_[2048], tlut_[2048];
 };
 
+// The float GPU renderer's inputs for one output frame (gpu.rs:754-957, core.rs:3235-3330):
+// the two source frames (planes at stride = plane width), the motion planes of the
+// motion grid (m[0..3] = motion0.x/y, motion1.x/y; m[4..7] = motion2.x/y, motion3.x/y for
+// algorithm 23), the coverage masks (algorithms >= 21) and the area (magnitude) masks.
+struct FloatFrame {
+    int width, height;
+    const uint8_t* s0[3];
+    const uint8_t* s1[3];
+    int mw, mh;
+    const uint16_t* m[8];
+    const uint8_t *cov0, *cov1, *mag0, *mag1;
+    bool has_sad;
+    int algo;                      // 21 for algorithm 23 without its neighbours
+    int phase;                     // the render threshold
+    int block_w, block_h;          // effective block (block - overlap)
+    int origin_x, origin_y;        // vector block / 2
+};
+void float_linear_lut(float lut[256]);                  // linear_luma kernel (gpu.rs)
+void float_render(const FloatFrame& f, uint8_t* dst[3]);  // render_frame kernel (gpu.rs:1102-1167)
+
 // ---------------------------------------------------------------- metadata / timing
 
 struct VectorData {  // metadata.rs:1-12 (vdata header)
@@ -151,6 +171,9 @@ struct SmoothClip {
     int super_rows;
     std::vector<const uint8_t*> vec;
     size_t vec_len;
+    // Rendered frames go through the float GPU renderer (gpu_render_frame, core.rs:1429,
+    // 1630-1653) where the reference would: interpolated vectors, no scene blending.
+    bool float_render = false;
 };
 
 int smooth_output_frames(const SmoothClip& c);

@@ -301,6 +301,32 @@ int smooth_render_frame(const SmoothClip& c, int k, uint8_t* dst[3], int* algo_o
         size_t pw = p ? (size_t)(c.width / 2) : (size_t)c.width, ph = p ? (size_t)(c.height / 2) : (size_t)c.height;
         d[p] = RPlaneMut{dst[p], pw * ph, pw};
     }
+    if (c.float_render && interp && !o.scene_blend) {
+        FloatFrame ff{};
+        ff.width = c.width;
+        ff.height = c.height;
+        for (int p = 0; p < 3; p++) {
+            ff.s0[p] = c.src[p][(size_t)n0];
+            ff.s1[p] = c.src[p][(size_t)n1];
+        }
+        ff.mw = (int)mw;
+        ff.mh = (int)mh;
+        const std::vector<uint16_t>* mp[8] = {&mx0, &my0, &mx1, &my1, &mx2, &my2, &mx3, &my3};
+        for (int q = 0; q < 8; q++) ff.m[q] = mp[q]->data();
+        ff.cov0 = cov0.data();
+        ff.cov1 = cov1.data();
+        ff.has_sad = o.mask_area_enabled;
+        ff.mag0 = mag0.data();
+        ff.mag1 = mag1.data();
+        ff.algo = (algo == 23 && !mode23_ready) ? 21 : algo;
+        ff.phase = phase;
+        ff.block_w = cfg.block_w;
+        ff.block_h = cfg.block_h;
+        ff.origin_x = cfg.origin_x;
+        ff.origin_y = cfg.origin_y;
+        float_render(ff, dst);
+        return 2;
+    }
     Motion m0{&mx0, &my0}, m1{&mx1, &my1}, m2{&mx2, &my2}, m3{&mx3, &my3};
     std::vector<uint8_t> maxm;
     if (o.mask_area_enabled) maxm = max_mask(mag0, mag1);

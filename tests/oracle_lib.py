@@ -172,7 +172,7 @@ L.oracle_smooth_opts.argtypes = [C.c_char_p, _sz, C.c_int64, C.c_int64, C.POINTE
                                  C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_char_p, _sz]
 L.oracle_smooth_render.argtypes = [C.c_char_p, _sz, _i, _i, C.c_int64, C.c_int64, _i, C.c_int64,
                                    C.POINTER(C.c_int32)] + [_pp] * 6 + [_i, _pp, _sz, _i, u8p, u8p, u8p,
-                                                                         C.POINTER(C.c_int32)]
+                                                                         C.POINTER(C.c_int32), _i]
 
 
 def smooth_opts(opt, fps_num, fps_den, vdata, nframes):
@@ -191,7 +191,8 @@ def smooth_opts(opt, fps_num, fps_den, vdata, nframes):
 class SmoothClip:
     """Host-side clip state for the oracle renderer: sources, super frames, vector payloads."""
 
-    def __init__(self, opt, frames, supers, payloads, sdata, vdata, fps=(24, 1)):
+    def __init__(self, opt, frames, supers, payloads, sdata, vdata, fps=(24, 1), float_render=False):
+        self.float_render = 1 if float_render else 0
         self.opt = (opt or "").encode()
         self.h, self.w = frames[0][0].shape
         self.n = len(frames)
@@ -224,7 +225,7 @@ class SmoothClip:
         rc = L.oracle_smooth_render(self.opt, len(self.opt), self.w, self.h, self.fps[0], self.fps[1], self.n,
                                     self.sdata, self.vdata, self.src[0], self.src[1], self.src[2], sup[0], sup[1],
                                     sup[2], self.super_rows, self.vec, self.vec_len, k, _p(Y), _p(U), _p(V),
-                                    C.byref(algo))
+                                    C.byref(algo), self.float_render)
         assert rc >= 0, rc
         return (Y, U, V), rc, algo.value
 

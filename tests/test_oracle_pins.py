@@ -177,3 +177,70 @@ def test_hierarchy_finds_integer_translation(oracle, search, pel):
         assert np.all(blk[..., 0] == sign * pel * sx), np.unique(blk[..., 0], return_counts=True)
         assert np.all(blk[..., 1] == sign * pel * sy), np.unique(blk[..., 1], return_counts=True)
         assert np.all(blk[..., 2] == 0)
+
+
+@pytest.mark.parametrize("algo", [1, 2, 11, 13, 21, 22, 23])
+def test_float_render_of_static_scene_is_the_source(oracle, algo):
+    """The float renderer (the reference's OpenCL kernel) on identical sources with zero
+    vectors: every blend of equal values is the value, and the luma's trip through linear
+    light and back rounds to the sample it started from."""
+    w, h = 128, 96
+    f = _frames(23, w, h)
+    frames = [f, f, f, f]
+    sopt = "{pel:2}"
+    so, _ = oracle.super_opts(sopt, w, h)
+    sups = [oracle.super_build(sopt, x) for x in frames]
+    aopt = "{block:{w:8,h:8,overlap:2},main:{search:{type:3,distance:8,satd:true}}}"
+    ao, _ = oracle.analyse_opts(aopt, so["sdata"], w, h)
+    pays = [oracle.analyse(aopt, so["sdata"], w, h, sups[i], sups[min(i + 1, 3)]) for i in range(4)]
+    ropt = "{rate:{num:5,den:2},algo:%d,mask:{area:100},scene:{mode:0,limits:{zero:0}}}" % algo
+    clip = oracle.SmoothClip(ropt, frames, sups, pays, so["sdata"], ao["vdata"], fps=(24, 1), float_render=True)
+    nout = oracle.smooth_opts(ropt, 24, 1, ao["vdata"], 4)[0]["out_frames"]
+    interp = 0
+    for k in range(nout):
+        out, kind, _ = clip.render(k)
+        interp += kind == 2
+        for got, want in zip(out, f):
+            assert np.array_equal(got, want), (algo, k, kind)
+    assert interp > 0
+
+
+def test_float_render_translation_moves_the_source(oracle):
+    """Algorithm 2 of the float renderer displaces source n by phase/256 of each vector
+    (in vector units; the reference's kernel takes pel as 1): a whole-pixel translation
+    at pel 1 renders, away from the borders, the source shifted by the fraction of the
+    shift the phase gives -- exact when that is a whole number of pixels."""
+    w, h, sx, sy = 160, 128, 8, -4
+    big = _frames(41, w + 32, h + 32)
+    a = tuple(p[8:8 + (h if i == 0 else h // 2), 8:8 + (w if i == 0 else w // 2)] for i, p in enumerate(big))
+    b = tuple(p[8 + (sy if i == 0 else sy // 2):8 + (sy if i == 0 else sy // 2) + (h if i == 0 else h // 2),
+                8 + (sx if i == 0 else sx // 2):8 + (sx if i == 0 else sx // 2) + (w if i == 0 else w // 2)]
+              for i, p in enumerate(big))
+    frames = [a, b, b, b]
+    sopt = "{pel:1}"
+    so, _ = oracle.super_opts(sopt, w, h)
+    sups = [oracle.super_build(sopt, x) for x in frames]
+    aopt = "{block:{w:16,h:16,overlap:0},main:{search:{type:4,distance:8,coarse:{type:4,distance:6}}," \
+           "lambda:0,penalty:{pnew:0,pzero:0,pglobal:0,pnbour:0}}}"
+    ao, _ = oracle.analyse_opts(aopt, so["sdata"], w, h)
+    pays = [oracle.analyse(aopt, so["sdata"], w, h, sups[i], sups[min(i + 1, 3)]) for i in range(4)]
+    # 2x: output 1 is at phase 128 of pair (0, 1); algorithm 2 samples frame 0 at x + v/2
+    ropt = "{rate:{num:2,den:1},algo:2,scene:{mode:0,limits:{zero:0}}}"
+    clip = oracle.SmoothClip(ropt, frames, sups, pays, so["sdata"], ao["vdata"], fps=(24, 1), float_render=True)
+    out, kind, algo = clip.render(1)
+    assert kind == 2 and algo == 2
+    hdr, regions = oracle.decode_payload(pays[0])
+    # set a (the backward region) holds (sx, sy) away from the borders: frame 0 at
+    # x + (sx, sy) / 2; the cubic vector sampling reaches two blocks, so the border
+    # blocks' vectors stay out of the compared area
+    m = 40
+    got = out[0][m:h - m, m:w - m]
+    want = a[0][m + sy // 2:h - m + sy // 2, m + sx // 2:w - m + sx // 2]
+    # luma goes through linear light and back (exact after rounding); chroma moves by
+    # half the luma shift
+    assert np.array_equal(got, want)
+    cm = m // 2
+    for p in (1, 2):
+        g = out[p][cm:h // 2 - cm, cm:w // 2 - cm]
+        q = a[p][cm + sy // 4:h // 2 - cm + sy // 4, cm + sx // 4:w // 2 - cm + sx // 4]
+        assert np.array_equal(g, q), p
