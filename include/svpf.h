@@ -155,6 +155,22 @@ int svpf_analyse_multi(svpf_ctx* ctx, int32_t n, const svpf_analyse_params* cons
  * the same vectors; 0 restores the default of 4096, the maximum. */
 int svpf_analyse_set_max_sweeps(svpf_ctx* ctx, int32_t max_sweeps);
 
+/* Runtime tuning of the Analyse level search (no reference counterpart; every setting
+ * gives the same vectors).  Zero fields keep the defaults.
+ *   chase        visit-order successors a later sweep evaluates right after a block
+ *                changes (default 0: none; measured slower on large batches, whose
+ *                early sweeps then wait on their longest chase);
+ *   slot_scale   factor on the finest level's sweep-0 slot count (default 1.0);
+ *   coarse_slots sweep-0 slots of the coarser levels (default 4096);
+ *   sweep_grid   workgroups of a later sweep (default 4096). */
+typedef struct svpf_analyse_tuning {
+    int32_t chase;
+    double slot_scale;
+    int32_t coarse_slots;
+    int32_t sweep_grid;
+} svpf_analyse_tuning;
+int svpf_analyse_set_tuning(svpf_ctx* ctx, const svpf_analyse_tuning* t);
+
 /* No counterpart in the reference.  svpf_analyse_pair for callers that cannot
  * allocate device memory (serving loops, harnesses that forbid hipMalloc): the same
  * arguments and payload, with the search's workspace, sort storage and descriptors

@@ -1261,6 +1261,7 @@ __global__ void sweep_init_kernel(const Chain* __restrict__ chains, SweepState s
         st.cpre[ch.base + i] = 1 << 30;  // sweep 0 runs no bad-block pass: the limit saturates
         st.memo[ch.base + i].valid = 0;
         st.seeds[ch.base + i].valid = 0;
+        st.stamp[ch.base + i] = 0;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st.minpos[blockIdx.y] = 0;  // the first rebuild covers the whole order
@@ -1381,9 +1382,10 @@ __global__ void __launch_bounds__(kScanThreads) sweep_scan_kernel(const Chain* _
 // kFast: the tile-plan evaluation; a block it passes on (whole, or with a provisional
 // result stored as any other) goes to the list of sweep s that deferred_kernel
 // completes before sweep s + 1.
+// Returns whether the block's stored vector changed; *me_out = its visit position.
 template <bool kFast>
-__device__ __forceinline__ void sweep_item(const Chain* __restrict__ chains, int ci, int idx, const SweepState& st,
-                                           int s) {
+__device__ __forceinline__ bool sweep_item(const Chain* __restrict__ chains, int ci, int idx, const SweepState& st,
+                                           int s, int* me_out = nullptr) {
     const Chain ch = uniform_load(&chains[ci]);
     const LevelGeom& g = ch.geom;
     Blk b;
@@ -1397,17 +1399,18 @@ __device__ __forceinline__ void sweep_item(const Chain* __restrict__ chains, int
                                               RefineStart{}, &defer);
     if (kFast && defer != kDeferNone) {
         if (lane_id() == 0) st.dl[atomicAdd(&st.dcount[s], 1u)] = ((unsigned long long)ci << 32) | (unsigned)idx;
-        if (defer == kDeferWhole) return;
+        if (defer == kDeferWhole) return false;
     }
     if (unchanged) {  // nothing it reads changed since its last evaluation
         if (lane_id() == 0) atomicAdd(&st.counts[st.max_sweeps + 1], 1u);  // skip count (statistics)
-        return;
+        return false;
     }
     const DVec old = uniform_load(&ch.res[idx]);
     const bool moved = o.dx != old.dx || o.dy != old.dy || o.score != old.score;
     const uint32_t oldf = (uint32_t)uniform_int((int)st.found[ch.base + idx]);
     const uint32_t oldb = (uint32_t)uniform_int((int)st.badf[ch.base + idx]);
     const int me = uniform_int(ch.opos[idx]);
+    if (me_out) *me_out = me;
     const int lane = lane_id();
     if (lane == 0) {
         if (moved || o.luma != old.luma || o.done != old.done) ch.res[idx] = o;
@@ -1419,7 +1422,7 @@ __device__ __forceinline__ void sweep_item(const Chain* __restrict__ chains, int
     }
     // sweep 0 queues nothing: validate_kernel compares every block's seed record with
     // its neighbours once the sweep is complete
-    if (!moved || s == 0) return;
+    if (!moved || s == 0) return moved;
     if (lane < 9 && lane != 4) {
         const int bx = idx % g.gw, by = idx / g.gw;
         const int nx = bx - 1 + lane % 3, ny = by - 1 + lane / 3;
@@ -1428,6 +1431,7 @@ __device__ __forceinline__ void sweep_item(const Chain* __restrict__ chains, int
             if (ch.opos[n] > me) queue_block(st, ci, ch, n, s);
         }
     }
+    return true;
 }
 
 // Position pos of the finest level's visit order, initial_search_order (:302-324): the
@@ -1524,9 +1528,36 @@ __global__ void validate_kernel(const Chain* __restrict__ chains, SweepState st,
     }
 }
 
-// Sweep s >= 1 over the blocks queued by sweep s - 1.  (Its lists are short after the
-// first few sweeps and each sweep waits on its slowest block, so a second, deferred
-// pass per sweep would cost more than three waves per SIMD gain.)
+// Claims block idx for an evaluation in sweep s (s >= 1): false when another wave of
+// this sweep already has it.  Two waves never evaluate one block in the same sweep.
+__device__ __forceinline__ bool claim_block(const SweepState& st, uint32_t base, int idx, int s) {
+    uint32_t old = 0;
+    if (lane_id() == 0) old = atomicMax(&st.stamp[base + idx], (uint32_t)s);
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)old) < (uint32_t)s;
+}
+
+// The block after visit position me, when it is a 3x3 neighbour of block idx (the one
+// block whose inputs a change of idx alters and that comes right after it); else -1.
+__device__ __forceinline__ int order_successor(const Chain* __restrict__ chains, int ci, int idx, int me) {
+    const LevelGeom* g = &chains[ci].geom;
+    const int gw = g->gw, gh = g->gh;
+    if (me + 1 >= gw * gh) return -1;
+    const int pk = uniform_int(chains[ci].order[me + 1]);
+    const int bx = pk & 0xFFFF, by = pk >> 16;
+    const int cx = idx % gw, cy = idx / gw;
+    if (bx >= gw || by >= gh || abs(bx - cx) > 1 || abs(by - cy) > 1) return -1;
+    return by * gw + bx;
+}
+
+// Sweep s >= 1 over the blocks queued by sweep s - 1.  A block whose vector changed has
+// its later neighbours queued for sweep s + 1; the one visited right after it (its
+// order successor) is also evaluated at once by the same wave, and so on along the
+// order while vectors change, up to st.chase blocks.  A change then travels along the
+// visit order within one sweep instead of one block per sweep.  The fixed point is the
+// same: a chased block read its predecessors' latest values, and any predecessor that
+// changes after that queues it for the next sweep.  (The lists are short after the first
+// few sweeps and each sweep waits on its slowest wave, so a second, deferred pass per
+// sweep would cost more than three waves per SIMD gain.)
 __global__ void __launch_bounds__(kWave, 2) sweep_kernel(const Chain* __restrict__ chains, SweepState st, int s) {
     const uint32_t n = (uint32_t)uniform_int((int)__atomic_load_n(&st.counts[s], __ATOMIC_RELAXED));
     const int p = s & 1;
@@ -1534,9 +1565,23 @@ __global__ void __launch_bounds__(kWave, 2) sweep_kernel(const Chain* __restrict
         const unsigned long long item = st.wl[p][i];
         // uniform across the wave: keeps the chain's descriptors in scalar registers
         const int ci = __builtin_amdgcn_readfirstlane((int)(item >> 32));
-        const int idx = __builtin_amdgcn_readfirstlane((int)(item & 0xFFFFFFFFu));
-        if (lane_id() == 0) st.flags[p][chains[ci].base + idx] = 0;
-        sweep_item<false>(chains, ci, idx, st, s);
+        int idx = __builtin_amdgcn_readfirstlane((int)(item & 0xFFFFFFFFu));
+        const uint32_t base = (uint32_t)uniform_int((int)chains[ci].base);
+        if (lane_id() == 0) st.flags[p][base + idx] = 0;
+        if (!claim_block(st, base, idx, s)) {  // a chase of this sweep evaluated it
+            if (lane_id() == 0) atomicAdd(&st.counts[st.max_sweeps + 1], 1u);
+            continue;
+        }
+        for (int k = 0;; k++) {
+            int me = 0;
+            if (!sweep_item<false>(chains, ci, idx, st, s, &me) || k >= st.chase) break;
+            const int next = order_successor(chains, ci, idx, me);
+            if (next < 0 || !claim_block(st, base, next, s)) break;
+            // this wave's store of idx before the successor reads it
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (lane_id() == 0) atomicAdd(&st.dcount[st.max_sweeps + 1], 1u);  // chased (statistics)
+            idx = next;
+        }
     }
 }
 
@@ -1558,9 +1603,11 @@ __global__ void __launch_bounds__(kWave, 2) deferred_kernel(const Chain* __restr
 }
 
 // Statistics of one converged level run (profiling): out4 += {1 run, `sweeps` launched,
-// evaluations skipped because nothing the block reads had changed (counts[max_sweeps +
-// 1]), evaluations run (every block in sweep 0 plus the lists of sweeps 1 .. sweeps - 1,
-// less the skipped ones)}.  Launches on one stream are ordered, so plain adds suffice.
+// list items not evaluated -- nothing the block reads had changed, or a chase of the
+// same sweep had evaluated it (counts[max_sweeps + 1]) --, evaluations run (every block
+// in sweep 0 plus the lists of sweeps 1 .. sweeps - 1 and the chased blocks
+// (dcount[max_sweeps + 1]), less those not evaluated)}.  Launches on one stream are
+// ordered, so plain adds suffice.
 __global__ void __launch_bounds__(256) sweep_stats_kernel(SweepState st, int sweeps, long long total_blocks,
                                                           unsigned long long* __restrict__ out4) {
     __shared__ unsigned long long part[256];
@@ -1574,10 +1621,11 @@ __global__ void __launch_bounds__(256) sweep_stats_kernel(SweepState st, int swe
     }
     if (threadIdx.x == 0) {
         const unsigned long long skipped = st.counts[st.max_sweeps + 1];
+        const unsigned long long chased = st.dcount[st.max_sweeps + 1];
         out4[0] += 1;
         out4[1] += (unsigned long long)sweeps;
         out4[2] += skipped;
-        out4[3] += (unsigned long long)total_blocks + part[0] - skipped;
+        out4[3] += (unsigned long long)total_blocks + part[0] + chased - skipped;
     }
 }
 
@@ -1955,7 +2003,7 @@ hipError_t launch_sweep_init(const Chain* d_chains, int nchains, int max_n, cons
 }
 
 hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s,
-                         int level, bool fast) {
+                         int level, bool fast, double slot_scale, int coarse_slots) {
     // segments per chain: a multiple of q so that the slot count is a multiple of the 8
     // XCDs, and no segment much shorter than seg_min blocks.  Fewer slots put more items
     // of a slot in flight at once (read before their predecessor is stored); more slots
@@ -1971,7 +2019,7 @@ hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const Sw
     // faster there, and twice the latency of a small batch's sweep 0).
     const double r = 32.0 / nchains;
     const double t0 = fast ? 3584.0 * std::pow(r, 0.8) : 2048.0 * std::sqrt(r);
-    const int target = level == 0 ? (int)std::min(8192.0, std::max(256.0, t0)) : 4096;
+    const int target = level == 0 ? (int)std::min(8192.0, std::max(256.0, t0 * slot_scale)) : coarse_slots;
     const int run = level == 0 ? 1 : 8;
     const int seg_min = fast ? 1024 : 2048;
     int g8 = 8;

@@ -102,7 +102,9 @@ struct SweepState {
     unsigned long long* dl;     // blocks the tile-plan kernels passed on in the current sweep
     uint32_t* dcount;           // dcount[s] = their number in sweep s
     uint32_t* fseq;             // per block: pre-expansion costs in visit order (count rebuild)
+    uint32_t* stamp;            // per block: the last sweep (>= 1) whose evaluation claimed it
     int max_sweeps;
+    int chase;                  // visit-order successors a later sweep evaluates right after a change
 };
 
 

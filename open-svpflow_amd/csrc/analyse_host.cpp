@@ -13,7 +13,7 @@ namespace svpf {
 hipError_t launch_search_level(const Chain* d_chains, int nchains, hipStream_t s);
 hipError_t launch_sweep_init(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s);
 hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s,
-                         int level, bool fast);
+                         int level, bool fast, double slot_scale, int coarse_slots);
 hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s);
 hipError_t launch_sweep_stats(const SweepState& st, int sweeps, long long total_blocks, unsigned long long* out4,
                               hipStream_t s);
@@ -174,8 +174,8 @@ int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total
         ctx->h_count_cap = words;
     }
     if (!no_alloc) h_count = ctx->h_count;
-    const int grid = (int)std::min<size_t>(std::max<size_t>(total_blocks, 1), 4096);
-    SVPF_HIP(launch_sweep0(dch, nch, max_n, st, ctx->stream, level, fast));
+    const int grid = (int)std::min<size_t>(std::max<size_t>(total_blocks, 1), (size_t)ctx->sweep_grid);
+    SVPF_HIP(launch_sweep0(dch, nch, max_n, st, ctx->stream, level, fast, ctx->slot_scale, ctx->coarse_slots));
     if (fast) SVPF_HIP(launch_deferred(dch, grid, st, 0, ctx->stream));
     if (scan) SVPF_HIP(launch_sweep_scan(dch, nch, st, 0, ctx->stream));
     SVPF_HIP(launch_validate(dch, nch, max_n, st, ctx->stream));
@@ -331,7 +331,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
     }
     // sweep state per level: flags x2 (one entry per block of every chain), worklists x2, counters
     size_t level_blocks[16] = {}, flags_off[16][2], wl_off[16][2], counts_off[16], found_off[16], badf_off[16],
-        cpre_off[16], minpos_off[16], memo_off[16], fseq_off[16], seed_off[16], dl_off[16];
+        cpre_off[16], minpos_off[16], memo_off[16], fseq_off[16], seed_off[16], dl_off[16], stamp_off[16];
     int level_max_n[16] = {};
     for (int l = 0; l < max_levels; l++) {
         size_t tot = 0;
@@ -357,6 +357,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
         memo_off[l] = work.take(sizeof(Memo) * (tot + 1));
         fseq_off[l] = work.take(4 * tot + 4);
         seed_off[l] = work.take(sizeof(SeedRec) * (tot + 1));
+        stamp_off[l] = work.take(4 * tot + 4);
     }
     for (int l = 0; l < max_levels; l++) {
         size_t tot = 0;
@@ -706,6 +707,8 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
             st.memo = (Memo*)(W + memo_off[l]);
             st.fseq = (uint32_t*)(W + fseq_off[l]);
             st.seeds = (SeedRec*)(W + seed_off[l]);
+            st.stamp = (uint32_t*)(W + stamp_off[l]);
+            st.chase = ctx->chase;
             st.max_sweeps = ctx->max_sweeps > 0 ? ctx->max_sweeps : kMaxSweeps;
             SVPF_HIP(launch_sweep_init(dch, nch_step[l], level_max_n[l], st, ctx->stream));
             int rc2 = seq_step[l] ? SVPF_OK
@@ -872,6 +875,18 @@ int svpf_analyse_batch(svpf_ctx* ctx, const svpf_analyse_params* p, int32_t n, c
 int svpf_analyse_pair(svpf_ctx* ctx, const svpf_analyse_params* p, const svpf_frame* d_cur, const svpf_frame* d_ref,
                       uint8_t* d_payload) {
     return svpf_analyse_batch(ctx, p, 1, d_cur, d_ref, &d_payload);
+}
+
+int svpf_analyse_set_tuning(svpf_ctx* ctx, const svpf_analyse_tuning* t) {
+    if (!ctx || !t) return fail(SVPF_ERR_ARG, "svpf_analyse_set_tuning: null argument");
+    if (t->chase > 65536 || t->slot_scale < 0.0 || t->slot_scale > 16.0 || t->coarse_slots < 0 ||
+        t->coarse_slots > (1 << 20) || t->sweep_grid < 0 || t->sweep_grid > (1 << 20))
+        return fail(SVPF_ERR_ARG, "svpf_analyse_set_tuning: setting out of range");
+    ctx->chase = std::max(t->chase, 0);
+    ctx->slot_scale = t->slot_scale == 0.0 ? 1.0 : t->slot_scale;
+    ctx->coarse_slots = t->coarse_slots == 0 ? 4096 : t->coarse_slots;
+    ctx->sweep_grid = t->sweep_grid == 0 ? 4096 : t->sweep_grid;
+    return SVPF_OK;
 }
 
 int svpf_analyse_set_max_sweeps(svpf_ctx* ctx, int32_t max_sweeps) {

@@ -28,6 +28,12 @@ struct svpf_ctx {
     size_t h_count_cap = 0;
     // Analyse: parallel sweeps per level before the sequential fallback (0 = the default).
     int max_sweeps = 0;
+    // Analyse: how many visit-order successors a later sweep evaluates at once after a
+    // block changes (0: none, every change waits for the next sweep).
+    int chase = 0;
+    double slot_scale = 1.0;   // factor on the finest level's sweep-0 slot count
+    int coarse_slots = 4096;   // sweep-0 slots of the coarser levels
+    int sweep_grid = 4096;     // workgroups of a later sweep
     // Analyse sweep statistics per level since profiling was last enabled (device, 16 x 4,
     // added by a kernel so that profiling adds no host round trip):
     // {level runs, sweeps launched, evaluations skipped, evaluations run}

@@ -24,6 +24,11 @@ class SvpfError(RuntimeError):
     pass
 
 
+class AnalyseTuning(C.Structure):
+    _fields_ = [("chase", C.c_int32), ("slot_scale", C.c_double), ("coarse_slots", C.c_int32),
+                ("sweep_grid", C.c_int32)]
+
+
 class Frame(C.Structure):
     _fields_ = [("y", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p),
                 ("stride_y", C.c_int32), ("stride_u", C.c_int32), ("stride_v", C.c_int32)]
@@ -71,6 +76,7 @@ def lib() -> C.CDLL:
         L.svpf_profile_read.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
         L.svpf_analyse_stats.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64)]
         L.svpf_analyse_set_max_sweeps.argtypes = [C.c_void_p, C.c_int32]
+        L.svpf_analyse_set_tuning.argtypes = [C.c_void_p, C.POINTER(AnalyseTuning)]
         for bind in _BINDERS:
             bind(L)
         _lib = L
@@ -167,6 +173,11 @@ class Context:
     def set_max_sweeps(self, n: int) -> None:
         """Parallel sweeps per Analyse level before the sequential fallback (0 = default)."""
         check(lib().svpf_analyse_set_max_sweeps(self.h, n))
+
+    def set_tuning(self, chase: int = 0, slot_scale: float = 0.0, coarse_slots: int = 0, sweep_grid: int = 0):
+        """svpf_analyse_set_tuning: zero keeps a default (chase 0: no chase)."""
+        t = AnalyseTuning(chase, slot_scale, coarse_slots, sweep_grid)
+        check(lib().svpf_analyse_set_tuning(self.h, C.byref(t)))
 
     def analyse_stats(self, level: int):
         """{runs, sweeps launched, evaluations skipped (inputs unchanged), evaluations run} of one level."""

@@ -148,3 +148,17 @@ def test_analyse_sequential_fallback(gpu_ctx, oracle, max_sweeps):
         gpu_ctx.profile(False)
     with pytest.raises(svpf.SvpfError):
         gpu_ctx.set_max_sweeps(5000)
+
+
+@pytest.mark.parametrize("tuning", [dict(chase=16), dict(chase=1), dict(chase=4096, sweep_grid=256),
+                                    dict(slot_scale=0.25, coarse_slots=512), dict(slot_scale=4.0, sweep_grid=64)])
+def test_analyse_tuning_keeps_vectors(gpu_ctx, oracle, tuning):
+    """The search's schedule settings (chase along the visit order, slot counts, sweep
+    grid) change only how the level's fixed point is reached, not the vectors."""
+    print(f"analyse parity: tuning {tuning}", flush=True)
+    frames = synth.clip(480, 272, 4, seed=2025, vmax=12.0, nobj=(3, 5), rot_deg=2.0)
+    gpu_ctx.set_tuning(**tuning)
+    try:
+        _check(gpu_ctx, oracle, frames, "{pel:2}", CFG2, batch=True)
+    finally:
+        gpu_ctx.set_tuning()
