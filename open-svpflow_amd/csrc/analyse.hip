@@ -682,18 +682,34 @@ struct RefineStart {
 // seed from it), no memo entry is stored and the seed record is invalidated.
 constexpr int kDeferNone = 0, kDeferWhole = 1, kDeferExact = 2;
 
+// The sweep state's words of one block (set by a sweep's evaluation; nullptr-free) and,
+// on return, what the evaluation read of them and of the block's previous result.
+struct BlockIO {
+    const int32_t* cpre;   // running bad-block count before the block
+    const uint32_t* found;
+    const uint32_t* badf;
+    bool fresh;            // sweep 0: the memo and seed record are known invalid
+    DVec old;              // out: the block's stored result before this evaluation
+    uint32_t oldf, oldb;   // out: its found / badf words
+    int me;                // out: its visit position
+};
+
+// Lanes of a block's gathered inputs; lanes 0-8 read the 3x3 neighbours.
+enum : int {
+    kInMe = 9, kInPred = 10, kInPrev = 12, kInGlob = 14, kInMemo = 16, kInRecBad = 22, kInRecValid = 23,
+    kInOld = 24, kInFound = 28, kInBadf = 29, kInCpre = 30,
+};
+
 template <bool kRefine, bool kFast = false>
 __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g, Blk& b, int idx,
                                              int bad_count, bool* bad, uint32_t* found_out, Memo* memo,
                                              SeedRec* rec, bool* unchanged, const RefineStart& rs,
-                                             int* defer = nullptr) {
+                                             int* defer = nullptr, BlockIO* io = nullptr) {
     const int lane = lane_id();
     const int part = lane & (kGroup - 1);
-    const int gx = ch.global ? uniform_int(ch.global[0]) : 0, gy = ch.global ? uniform_int(ch.global[1]) : 0;
     const int gw = g.gw, gh = g.gh, level = g.level;
     const bool smallest = !kRefine && g.smallest != 0;
     const int bx = idx % gw, by = idx / gw;
-    const int me = kRefine ? 0 : uniform_int(ch.opos[idx]);
     int px, py, bw, bh;
     if constexpr (kRefine) {
         px = rs.px; py = rs.py; bw = rs.bw; bh = rs.bh;
@@ -711,6 +727,69 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
             return DVec{};
         }
     }
+    // the tile plan of the block at its own position (its source rows load here, in
+    // flight with the inputs below); the edge predictor's measure uses group_cost
+    TilePlan tp;
+    tp.on = 0;
+    if (!edge || smallest) {
+        make_plan(b, !smallest, part, tp);
+        if constexpr (kFast) {
+            if (!(tp.on && tp.dc)) {
+                *defer = kDeferWhole;
+                return DVec{};
+            }
+        }
+    }
+    // ---- the block's inputs in one round of loads: lane k < 9 reads its 3x3 neighbour
+    // (visit position, vector) and the seed record's copy of it; lanes 9-30 read one
+    // word each of the block's own inputs, taken below with readlane.
+    const bool fresh = io != nullptr && io->fresh;
+    uint32_t word = 0, nvec = 0;
+    int npos = INT32_MAX;
+    if constexpr (!kRefine) {
+        const uint32_t* own = reinterpret_cast<const uint32_t*>(ch.opos + idx);  // stand-in for absent words
+        const bool rec_ok = rec != nullptr && !fresh, memo_ok = memo != nullptr && !fresh;
+        const uint32_t* w = own;
+        if (lane < 9) w = rec_ok ? rec->seed + lane : own;
+        else if (lane < kInPred) w = own;
+        else if (lane < kInPrev) w = reinterpret_cast<const uint32_t*>(ch.in + idx) + (lane - kInPred);
+        else if (lane < kInGlob) w = ch.prev ? reinterpret_cast<const uint32_t*>(ch.prev + idx) + (lane - kInPrev) : own;
+        else if (lane < kInMemo) w = ch.global ? reinterpret_cast<const uint32_t*>(ch.global) + (lane - kInGlob) : own;
+        else if (lane < kInRecBad) w = memo_ok ? reinterpret_cast<const uint32_t*>(memo) + (lane - kInMemo) : own;
+        else if (lane == kInRecBad) w = rec_ok ? reinterpret_cast<const uint32_t*>(&rec->bad_count) : own;
+        else if (lane == kInRecValid) w = rec_ok ? &rec->valid : own;
+        else if (lane < kInFound) w = io ? reinterpret_cast<const uint32_t*>(ch.res + idx) + (lane - kInOld) : own;
+        else if (lane == kInFound) w = io ? io->found : own;
+        else if (lane == kInBadf) w = io ? io->badf : own;
+        else if (lane == kInCpre) w = io ? reinterpret_cast<const uint32_t*>(io->cpre) : own;
+        word = *w;
+        const int nx = bx - 1 + lane % 3, ny = by - 1 + lane / 3;
+        if (lane < 9 && nx >= 0 && ny >= 0 && nx < gw && ny < gh) {
+            const int n = ny * gw + nx;
+            npos = ch.opos[n];
+            // device-scope load: the neighbour may have been stored in this launch by
+            // another CU (its vector is the first word of the record)
+            nvec = __hip_atomic_load(reinterpret_cast<const uint32_t*>(&ch.res[n]), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    auto in_word = [&](int k) { return rlu(word, k); };
+    int gx = 0, gy = 0, me = 0;
+    if constexpr (kRefine) {
+        if (ch.global) { gx = uniform_int(ch.global[0]); gy = uniform_int(ch.global[1]); }
+    } else {
+        if (ch.global) { gx = (int)in_word(kInGlob); gy = (int)in_word(kInGlob + 1); }
+        me = (int)in_word(kInMe);
+        if (io) {
+            bad_count = (int)in_word(kInCpre);
+            io->me = me;
+            const uint32_t o0 = in_word(kInOld);
+            io->old = DVec{(int16_t)(o0 & 0xFFFFu), (int16_t)(o0 >> 16), in_word(kInOld + 1), in_word(kInOld + 2),
+                           in_word(kInOld + 3)};
+            io->oldf = in_word(kInFound);
+            io->oldb = in_word(kInBadf);
+        }
+    }
     DVec pred;
     int epx = 0, epy = 0;
     if constexpr (kRefine) {
@@ -726,7 +805,8 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
         epx = (si % gw) * g.step_x;
         epy = (si / gw) * g.step_y;
     } else {
-        pred = uniform_load(&ch.in[idx]);
+        const uint32_t p0 = in_word(kInPred);
+        pred = DVec{(int16_t)(p0 & 0xFFFFu), (int16_t)(p0 >> 16), in_word(kInPred + 1), 0u, 0u};
     }
     int radius = kRefine ? rs.radius : g.radius;
     if (!kRefine && level > 0 && g.adaptive) {
@@ -740,21 +820,15 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
     int nseed_lane = 0;
     int sdx = 0, sdy = 0;
     uint32_t seed_word = 0;
-    if (!kRefine && lane < 9) {
-        const int nx = bx - 1 + lane % 3, ny = by - 1 + lane / 3;
-        if (nx >= 0 && ny >= 0 && nx < gw && ny < gh && ch.opos[ny * gw + nx] < me) {
-            // device-scope load: the neighbour may have been stored in this launch by
-            // another CU (its vector is the first word of the record)
-            seed_word = __hip_atomic_load(reinterpret_cast<const uint32_t*>(&ch.res[ny * gw + nx]),
-                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            nseed_lane = 1; sdx = (int16_t)(seed_word & 0xFFFFu); sdy = (int16_t)(seed_word >> 16);
-        }
+    if (!kRefine && lane < 9 && npos < me) {
+        seed_word = nvec;
+        nseed_lane = 1; sdx = (int16_t)(seed_word & 0xFFFFu); sdy = (int16_t)(seed_word >> 16);
     }
     // the same neighbours and bad-block count as the last evaluation: the same result
     if (edge) rec = nullptr;
-    if (rec) {
-        const bool differs = lane < 9 && nseed_lane && rec->seed[lane] != seed_word;
-        if (__ballot(differs) == 0 && uniform_int((int)rec->valid) != 0 && uniform_int(rec->bad_count) == bad_count) {
+    if (!kRefine && rec && !fresh) {
+        const bool differs = lane < 9 && nseed_lane && word != seed_word;  // word: the record's copy
+        if (__ballot(differs) == 0 && in_word(kInRecValid) != 0 && (int)in_word(kInRecBad) == bad_count) {
             *unchanged = true;
             return DVec{};
         }
@@ -767,9 +841,15 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
     bool has_prev = false;
     int prx = 0, pry = 0;
     if (ch.prev) {
-        const DVec pv = uniform_load(&ch.prev[idx]);
-        has_prev = pv.score != 0xFFFFFFFFu && pv.score > 0;
-        prx = pv.dx; pry = pv.dy;
+        if constexpr (kRefine) {
+            const DVec pv = uniform_load(&ch.prev[idx]);
+            has_prev = pv.score != 0xFFFFFFFFu && pv.score > 0;
+            prx = pv.dx; pry = pv.dy;
+        } else {
+            const uint32_t v0 = in_word(kInPrev), sc = in_word(kInPrev + 1);
+            has_prev = sc != 0xFFFFFFFFu && sc > 0;
+            prx = (int16_t)(v0 & 0xFFFFu); pry = (int16_t)(v0 >> 16);
+        }
     }
 
     // the search context; its predictor is the block's own (pred) except where the
@@ -803,8 +883,6 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
     Pattern P;
     bool pchroma;
     int step;
-    TilePlan tp;  // for the block at its own position (the edge predictor's measure uses group_cost)
-    tp.on = 0;
     if (edge && !smallest) {
         b.px = epx; b.py = epy;
         lact = lane == 0; lx = pred.dx; ly = pred.dy; lpen = 0;
@@ -812,13 +890,6 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
         pchroma = true;
         step = kStEdgeDone;
     } else {
-        make_plan(b, !smallest, part, tp);
-        if constexpr (kFast) {
-            if (!(tp.on && tp.dc)) {
-                *defer = kDeferWhole;
-                return DVec{};
-            }
-        }
         set_pred_list();
         // the refine pass measures only its start vector (lane 0's field entry)
         P = Pattern{kPatPred, 0, 0, 0, 0, kRefine ? 1 : 4 + nseeds};
@@ -1027,10 +1098,12 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                     best.cost = (int64_t)best.sad;
                     key_x = best.mvx;
                     key_y = best.mvy;
-                    if (memo) {
-                        const Memo m = uniform_load(memo);
-                        if (m.valid && m.kx == key_x && m.ky == key_y) {
-                            best = Best{m.rx, m.ry, m.rsad, m.rcost};
+                    if (memo && !fresh) {  // (gathered with the block's inputs)
+                        const uint32_t k0 = in_word(kInMemo), r0 = in_word(kInMemo + 1);
+                        if (in_word(kInMemo + 3) != 0 && (int16_t)(k0 & 0xFFFFu) == key_x &&
+                            (int16_t)(k0 >> 16) == key_y) {
+                            best = Best{(int16_t)(r0 & 0xFFFFu), (int16_t)(r0 >> 16), in_word(kInMemo + 2),
+                                        join64(in_word(kInMemo + 4), in_word(kInMemo + 5))};
                             memo_hit = true;
                             step = kStRefineDone;
                             break;
@@ -1394,9 +1467,13 @@ __device__ __forceinline__ bool sweep_item(const Chain* __restrict__ chains, int
     uint32_t found;
     bool unchanged = false;
     int defer = kDeferNone;
-    const DVec o = search_block<false, kFast>(ch, g, b, idx, uniform_int(st.cpre[ch.base + idx]), &bad, &found,
-                                              st.memo + ch.base + idx, st.seeds + ch.base + idx, &unchanged,
-                                              RefineStart{}, &defer);
+    BlockIO io;
+    io.cpre = st.cpre + ch.base + idx;
+    io.found = st.found + ch.base + idx;
+    io.badf = st.badf + ch.base + idx;
+    io.fresh = s == 0;  // sweep_init cleared every memo and seed record of the level
+    const DVec o = search_block<false, kFast>(ch, g, b, idx, 0, &bad, &found, st.memo + ch.base + idx,
+                                              st.seeds + ch.base + idx, &unchanged, RefineStart{}, &defer, &io);
     if (kFast && defer != kDeferNone) {
         if (lane_id() == 0) st.dl[atomicAdd(&st.dcount[s], 1u)] = ((unsigned long long)ci << 32) | (unsigned)idx;
         if (defer == kDeferWhole) return false;
@@ -1405,11 +1482,11 @@ __device__ __forceinline__ bool sweep_item(const Chain* __restrict__ chains, int
         if (lane_id() == 0) atomicAdd(&st.counts[st.max_sweeps + 1], 1u);  // skip count (statistics)
         return false;
     }
-    const DVec old = uniform_load(&ch.res[idx]);
+    // (the block's own words: no other evaluation stores them while this one runs)
+    const DVec old = io.old;
     const bool moved = o.dx != old.dx || o.dy != old.dy || o.score != old.score;
-    const uint32_t oldf = (uint32_t)uniform_int((int)st.found[ch.base + idx]);
-    const uint32_t oldb = (uint32_t)uniform_int((int)st.badf[ch.base + idx]);
-    const int me = uniform_int(ch.opos[idx]);
+    const uint32_t oldf = io.oldf, oldb = io.oldb;
+    const int me = io.me;
     if (me_out) *me_out = me;
     const int lane = lane_id();
     if (lane == 0) {
