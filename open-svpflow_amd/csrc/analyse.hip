@@ -700,15 +700,19 @@ enum : int {
     kInOld = 24, kInFound = 28, kInBadf = 29, kInCpre = 30,
 };
 
-template <bool kRefine, bool kFast = false>
+// kL0: the finest level of a multi-level search with pnew >= -256, as compile-time
+// facts (no edge row/column, no half-size block, no bad-block pass, no adaptive radius,
+// no serial replay), so that the instantiation carries none of their state.
+template <bool kRefine, bool kFast = false, bool kL0 = false>
 __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g, Blk& b, int idx,
                                              int bad_count, bool* bad, uint32_t* found_out, Memo* memo,
                                              SeedRec* rec, bool* unchanged, const RefineStart& rs,
                                              int* defer = nullptr, BlockIO* io = nullptr) {
     const int lane = lane_id();
     const int part = lane & (kGroup - 1);
-    const int gw = g.gw, gh = g.gh, level = g.level;
-    const bool smallest = !kRefine && g.smallest != 0;
+    const int gw = g.gw, gh = g.gh, level = kL0 ? 0 : g.level;
+    const bool smallest = !kRefine && !kL0 && g.smallest != 0;
+    if constexpr (kL0) b.level = 0;
     const int bx = idx % gw, by = idx / gw;
     int px, py, bw, bh;
     if constexpr (kRefine) {
@@ -857,7 +861,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
     Search cur;
     cur.pnew = kRefine ? rs.pnew : g.pnew;
     cur.chroma = !smallest;
-    cur.serial = cur.pnew < -256;
+    cur.serial = !kL0 && cur.pnew < -256;
     cur.bnd[0] = -g.pel * px; cur.bnd[1] = -g.pel * py;
     cur.bnd[2] = g.pel * (g.lw - px - bw) + 1; cur.bnd[3] = g.pel * (g.lh - py - bh) + 1;
     cur.pmx = pred.dx; cur.pmy = pred.dy;
@@ -1456,7 +1460,7 @@ __global__ void __launch_bounds__(kScanThreads) sweep_scan_kernel(const Chain* _
 // result stored as any other) goes to the list of sweep s that deferred_kernel
 // completes before sweep s + 1.
 // Returns whether the block's stored vector changed; *me_out = its visit position.
-template <bool kFast>
+template <bool kFast, bool kL0 = false>
 __device__ __forceinline__ bool sweep_item(const Chain* __restrict__ chains, int ci, int idx, const SweepState& st,
                                            int s, int* me_out = nullptr) {
     const Chain ch = uniform_load(&chains[ci]);
@@ -1472,7 +1476,7 @@ __device__ __forceinline__ bool sweep_item(const Chain* __restrict__ chains, int
     io.found = st.found + ch.base + idx;
     io.badf = st.badf + ch.base + idx;
     io.fresh = s == 0;  // sweep_init cleared every memo and seed record of the level
-    const DVec o = search_block<false, kFast>(ch, g, b, idx, 0, &bad, &found, st.memo + ch.base + idx,
+    const DVec o = search_block<false, kFast, kL0>(ch, g, b, idx, 0, &bad, &found, st.memo + ch.base + idx,
                                               st.seeds + ch.base + idx, &unchanged, RefineStart{}, &defer, &io);
     if (kFast && defer != kDeferNone) {
         if (lane_id() == 0) st.dl[atomicAdd(&st.dcount[s], 1u)] = ((unsigned long long)ci << 32) | (unsigned)idx;
@@ -1544,7 +1548,7 @@ __device__ __forceinline__ void level0_order(int pos, int gw, int gh, int& x, in
 // wave.  The coarse levels' order lists each sorted parent's children together, so with
 // runs of 8 (two parents) a block's earlier siblings are mostly stored before it reads
 // them; the finest level's meander is served best by single positions.
-template <bool kFast, int kRun>
+template <bool kFast, int kRun, bool kL0 = false>
 __global__ void __launch_bounds__(kWave, kFast ? 3 : 2) sweep0_kernel(const Chain* __restrict__ chains,
                                                                      SweepState st, int nslots, int nseg,
                                                                      long long total) {
@@ -1567,7 +1571,7 @@ __global__ void __launch_bounds__(kWave, kFast ? 3 : 2) sweep0_kernel(const Chai
                 by = pk >> 16;
             }
             if (bx >= gw || by >= gh) continue;
-            sweep_item<kFast>(chains, ci, by * g->gw + bx, st, 0);
+            sweep_item<kFast, kL0>(chains, ci, by * g->gw + bx, st, 0);
             // the next position's seed reads see this block's result
             if (kRun > 1) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         }
@@ -1635,6 +1639,7 @@ __device__ __forceinline__ int order_successor(const Chain* __restrict__ chains,
 // changes after that queues it for the next sweep.  (The lists are short after the first
 // few sweeps and each sweep waits on its slowest wave, so a second, deferred pass per
 // sweep would cost more than three waves per SIMD gain.)
+template <bool kL0>
 __global__ void __launch_bounds__(kWave, 2) sweep_kernel(const Chain* __restrict__ chains, SweepState st, int s) {
     const uint32_t n = (uint32_t)uniform_int((int)__atomic_load_n(&st.counts[s], __ATOMIC_RELAXED));
     const int p = s & 1;
@@ -1651,7 +1656,7 @@ __global__ void __launch_bounds__(kWave, 2) sweep_kernel(const Chain* __restrict
         }
         for (int k = 0;; k++) {
             int me = 0;
-            if (!sweep_item<false>(chains, ci, idx, st, s, &me) || k >= st.chase) break;
+            if (!sweep_item<false, kL0>(chains, ci, idx, st, s, &me) || k >= st.chase) break;
             const int next = order_successor(chains, ci, idx, me);
             if (next < 0 || !claim_block(st, base, next, s)) break;
             // this wave's store of idx before the successor reads it
@@ -2080,7 +2085,7 @@ hipError_t launch_sweep_init(const Chain* d_chains, int nchains, int max_n, cons
 }
 
 hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s,
-                         int level, bool fast, double slot_scale, int coarse_slots) {
+                         int level, bool fast, bool l0, double slot_scale, int coarse_slots) {
     // segments per chain: a multiple of q so that the slot count is a multiple of the 8
     // XCDs, and no segment much shorter than seg_min blocks.  Fewer slots put more items
     // of a slot in flight at once (read before their predecessor is stored); more slots
@@ -2112,7 +2117,10 @@ hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const Sw
     const int nslots = nchains * nseg;
     const long long total = (long long)nslots * (((max_n + nseg - 1) / nseg + run - 1) / run);
     const unsigned grid = (unsigned)std::min<long long>(total, 1 << 22);
-    if (fast && run == 1)
+    if (fast && run == 1 && l0)
+        hipLaunchKernelGGL((sweep0_kernel<true, 1, true>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg,
+                           total);
+    else if (fast && run == 1)
         hipLaunchKernelGGL((sweep0_kernel<true, 1>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
     else if (fast)
         hipLaunchKernelGGL((sweep0_kernel<true, 8>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
@@ -2128,8 +2136,11 @@ hipError_t launch_refine(const void* d_jobs, int njobs, int max_n, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s) {
-    hipLaunchKernelGGL(sweep_kernel, dim3(grid), dim3(kWave), 0, s, d_chains, st, sweep);
+hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s, bool l0) {
+    if (l0)
+        hipLaunchKernelGGL(sweep_kernel<true>, dim3(grid), dim3(kWave), 0, s, d_chains, st, sweep);
+    else
+        hipLaunchKernelGGL(sweep_kernel<false>, dim3(grid), dim3(kWave), 0, s, d_chains, st, sweep);
     return hipGetLastError();
 }
 

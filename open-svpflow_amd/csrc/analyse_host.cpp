@@ -13,8 +13,8 @@ namespace svpf {
 hipError_t launch_search_level(const Chain* d_chains, int nchains, hipStream_t s);
 hipError_t launch_sweep_init(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s);
 hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s,
-                         int level, bool fast, double slot_scale, int coarse_slots);
-hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s);
+                         int level, bool fast, bool l0, double slot_scale, int coarse_slots);
+hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s, bool l0);
 hipError_t launch_sweep_stats(const SweepState& st, int sweeps, long long total_blocks, unsigned long long* out4,
                               hipStream_t s);
 hipError_t launch_deferred(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s);
@@ -162,7 +162,7 @@ constexpr int kNotConverged = 1;
 // chain's searched block is a 4x4 or 8x8 square, and sweep 0 (every block of the level)
 // runs the tile-plan kernel, then the general one over the blocks it passed on.
 int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total_blocks, const SweepState& st,
-               bool scan, int level, bool fast, bool no_alloc) {
+               bool scan, int level, bool fast, bool l0, bool no_alloc) {
     const size_t words = (size_t)kMaxSweeps + 2;
     uint32_t local_count = 0;  // no_alloc: read back through pageable memory (a synchronous copy)
     uint32_t* h_count = no_alloc ? &local_count : nullptr;
@@ -175,7 +175,7 @@ int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total
     }
     if (!no_alloc) h_count = ctx->h_count;
     const int grid = (int)std::min<size_t>(std::max<size_t>(total_blocks, 1), (size_t)ctx->sweep_grid);
-    SVPF_HIP(launch_sweep0(dch, nch, max_n, st, ctx->stream, level, fast, ctx->slot_scale, ctx->coarse_slots));
+    SVPF_HIP(launch_sweep0(dch, nch, max_n, st, ctx->stream, level, fast, l0, ctx->slot_scale, ctx->coarse_slots));
     if (fast) SVPF_HIP(launch_deferred(dch, grid, st, 0, ctx->stream));
     if (scan) SVPF_HIP(launch_sweep_scan(dch, nch, st, 0, ctx->stream));
     SVPF_HIP(launch_validate(dch, nch, max_n, st, ctx->stream));
@@ -183,7 +183,7 @@ int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total
     for (;;) {
         if (s + chunk > st.max_sweeps) return kNotConverged;
         for (int k = 0; k < chunk; k++) {
-            SVPF_HIP(launch_sweep(dch, grid, st, s + k, ctx->stream));
+            SVPF_HIP(launch_sweep(dch, grid, st, s + k, ctx->stream, l0));
             if (scan) SVPF_HIP(launch_sweep_scan(dch, nch, st, s + k, ctx->stream));
         }
         s += chunk;
@@ -464,7 +464,9 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
     int nch_step[16] = {}, ntr_step[16] = {}, nsort_step[16] = {}, max_tr_n[16] = {}, max_sort_n[16] = {};
     bool seq_step[16] = {}, bad_step[16] = {};  // sequential fallback / bad-block count rebuilds
     bool plan_step[16];                         // every searched block a 4x4 or 8x8 square
+    bool l0_step[16];                           // the finest level of multi-level searches, pnew >= -256
     for (bool& f : plan_step) f = true;
+    for (int l = 0; l < 16; l++) l0_step[l] = l == 0;
     Transition* inits = (Transition*)&host[init_off];
     int ninit = 0, max_init_n = 0;
     for (int step = max_levels - 1; step >= 0; step--) {
@@ -567,6 +569,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
                 const LevelGeom& g = c.geom[l];
                 const int sw = g.smallest ? g.bw >> 1 : g.bw, sh = g.smallest ? g.bh >> 1 : g.bh;
                 if (sw != sh || (sw != 4 && sw != 8)) plan_step[l] = false;
+                if (g.level != 0 || g.smallest || g.pnew < -256) l0_step[l] = false;
             }
         }
     }
@@ -713,7 +716,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
             SVPF_HIP(launch_sweep_init(dch, nch_step[l], level_max_n[l], st, ctx->stream));
             int rc2 = seq_step[l] ? SVPF_OK
                                   : run_sweeps(ctx, dch, nch_step[l], level_max_n[l], level_blocks[l], st, bad_step[l],
-                                               l, plan_step[l], ext != nullptr);
+                                               l, plan_step[l], l0_step[l], ext != nullptr);
             if (rc2 == kNotConverged || seq_step[l])
                 SVPF_HIP(launch_search_level(dch, nch_step[l], ctx->stream));
             else if (rc2)
