@@ -41,6 +41,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "open-svpflow_amd"))
 
 import svpf  # noqa: E402
+import autotune  # noqa: E402
 import synth  # noqa: E402
 
 SUPER = "{pel:2}"
@@ -336,12 +337,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--frames", type=int, default=120)
     ap.add_argument("--plain-frames", type=int, default=24, help="source frames of the per-frame path sample")
-    ap.add_argument("--cpu-pairs", type=int, default=8, help="frame pairs of the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-pairs", type=int, default=64, help="frame pairs of the CPU baseline sample (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=15.0,
                     help="length of the config-5 serving run that gives the p50/p99 frame latency (0: skip)")
     ap.add_argument("--config4-clips", type=int, default=64, help="clips of the config-4 batched run (0: skip)")
     ap.add_argument("--config4-pairs-per-call", type=int, default=480,
                     help="frame pairs per svpf_analyse_multi call of the config-4 run (whole clips)")
+    ap.add_argument("--tune-seconds", type=float, default=45.0,
+                    help="wall-time budget of the Analyse schedule autotuner on the headline clip (0: defaults)")
+    ap.add_argument("--config4-tune-seconds", type=float, default=30.0,
+                    help="the same for the config-4 batch, on its first svpf_analyse_multi call (0: keep)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run 4 device-to-device copies of MIB MiB each (the known-size reference for "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE scaling)")
@@ -373,6 +378,15 @@ def main():
         ctx.free(b)
         print(f"[bench] copy calibration: 4 x {nb} bytes device to device", file=sys.stderr, flush=True)
     clip = Clip(ctx, frames)
+    tuning = None
+    if args.tune_seconds > 0:
+        # the search schedule for this batch, measured (untimed; every setting gives the same vectors)
+        print(f"[bench] autotune of the Analyse schedule, {args.tune_seconds:.0f} s budget", file=sys.stderr, flush=True)
+        svpf.super_build_batch(ctx, clip.sp, clip.src, clip.sup)
+        res = autotune.tune(ctx, lambda: svpf.analyse_batch(ctx, clip.ap, clip.sup, clip.refs, clip.pay),
+                            [(p, clip.plen) for p in clip.pay], budget_s=args.tune_seconds)
+        tuning = res.as_dict()
+        print(f"[bench] autotune: {tuning}", file=sys.stderr, flush=True)
     print(f"[bench] warmup {args.warmup}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         clip.batched_step()
@@ -402,6 +416,7 @@ def main():
     if args.plain_frames > 1:
         nf = min(args.plain_frames, clip.n - 1)
         print(f"[bench] plain per-frame path, {nf} source frames", file=sys.stderr, flush=True)
+        autotune.apply(ctx, autotune.DEFAULT)  # one pair per call: the library's schedule
         sm = clip.plain_session(nf)
         clip.plain_pass(sm, nf)  # one untimed pass: the per-pair workspace and descriptors exist
         ctx.sync()
@@ -436,6 +451,25 @@ def main():
     if args.config4_clips > 0:
         print(f"[bench] config 4: {args.config4_clips} clips", file=sys.stderr, flush=True)
         clips = config4_setup(ctx, args.config4_clips)
+        c4_tuning = None
+        if args.config4_tune_seconds > 0:
+            for c in clips:
+                svpf.super_build_batch(ctx, c.sp, c.src, c.sup)
+            first, npairs = [], 0
+            for c in clips:
+                if first and npairs + c.n > args.config4_pairs_per_call:
+                    break
+                first.append(c)
+                npairs += c.n
+            print(f"[bench] config 4: autotune on its first call ({npairs} pairs), "
+                  f"{args.config4_tune_seconds:.0f} s budget", file=sys.stderr, flush=True)
+            res = autotune.tune(
+                ctx, lambda: svpf.analyse_multi(ctx, [x.ap for x in first for _ in range(x.n)],
+                                                [f for x in first for f in x.sup], [f for x in first for f in x.refs],
+                                                [q for x in first for q in x.pay]),
+                [(q, x.plen) for x in first for q in x.pay], budget_s=args.config4_tune_seconds)
+            c4_tuning = res.as_dict()
+            print(f"[bench] config 4 autotune: {c4_tuning}", file=sys.stderr, flush=True)
         config4_pass(ctx, clips, args.config4_pairs_per_call)  # warm-up (workspace, sessions' prep buffers)
         ctx.sync()
         t0 = time.perf_counter()
@@ -446,7 +480,7 @@ def main():
         by_res = {f"{w}x{h}": sum(1 for c in clips if (c.w, c.h) == (w, h)) for (w, h) in C4_RES}
         c4 = {"value": round(nout / dt, 3), "unit": "fps", "seconds": round(dt, 3), "clips": len(clips),
               "source_frames": sum(c.n for c in clips), "output_frames": nout, "clips_by_size": by_res,
-              "pairs_per_analyse_call": args.config4_pairs_per_call,
+              "pairs_per_analyse_call": args.config4_pairs_per_call, "analyse_tuning": c4_tuning,
               "source_megapixels_per_s": round(sum(c.n * c.w * c.h for c in clips) / dt / 1e6, 1)}
         for c in clips:
             c.sm.close()
@@ -466,6 +500,7 @@ def main():
         # steps are small batches, whose level searches are bound by latency, not throughput)
         src_rate = clip.n * clip.w * clip.h * args.steps / wall
         print(f"[bench] serving run, {args.serve_seconds:.0f} s of arrivals", file=sys.stderr, flush=True)
+        autotune.apply(ctx, autotune.DEFAULT)  # small batches: the library's schedule
         serving = serving_run(ctx, args.serve_seconds, 0.25 * src_rate)
 
     cpu = None
@@ -518,6 +553,7 @@ def main():
                            "frac": super_gbs / HBM_PEAK_GBS, "launches": super_n,
                            "avg_launch_ms": round(super_ms / max(super_n, 1), 3)},
         "stage_ms_per_step": {k: round(v[0] / args.steps, 2) for k, v in prof.items()},
+        "analyse_tuning": tuning,
         "analyse_sweeps": {f"level{lv}": v for lv, v in sweeps.items()},
         "config4": c4,
         "p99_frame_latency_ms": serving["p99_ms"] if serving else None,
