@@ -345,6 +345,8 @@ def main():
                     help="frame pairs per svpf_analyse_multi call of the config-4 run (whole clips)")
     ap.add_argument("--tune-seconds", type=float, default=45.0,
                     help="wall-time budget of the Analyse schedule autotuner on the headline clip (0: defaults)")
+    ap.add_argument("--tuning", type=str, default="", metavar="K=V,...",
+                    help="a fixed Analyse schedule instead of the autotuner, e.g. slot_scale=1.4,sweep_grid=2048")
     ap.add_argument("--config4-tune-seconds", type=float, default=30.0,
                     help="the same for the config-4 batch, on its first svpf_analyse_multi call (0: keep)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -379,7 +381,11 @@ def main():
         print(f"[bench] copy calibration: 4 x {nb} bytes device to device", file=sys.stderr, flush=True)
     clip = Clip(ctx, frames)
     tuning = None
-    if args.tune_seconds > 0:
+    if args.tuning:
+        fixed = {k: float(v) for k, v in (kv.split("=") for kv in args.tuning.split(","))}
+        autotune.apply(ctx, fixed)
+        tuning = {"fixed": fixed}
+    elif args.tune_seconds > 0:
         # the search schedule for this batch, measured (untimed; every setting gives the same vectors)
         print(f"[bench] autotune of the Analyse schedule, {args.tune_seconds:.0f} s budget", file=sys.stderr, flush=True)
         svpf.super_build_batch(ctx, clip.sp, clip.src, clip.sup)

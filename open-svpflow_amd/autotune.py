@@ -9,6 +9,8 @@ settings only decide how fast it gets there:
   predecessor is stored); both cost re-evaluations in the later sweeps.
 * ``coarse_slots`` -- sweep-0 slots of the coarser levels.
 * ``sweep_grid``   -- workgroups of a later sweep (a persistent grid over the queue).
+* ``fine_run``     -- consecutive finest-level positions one sweep-0 item searches in
+  order (a block's reference rows stay cached for the next).
 
 Their best values move with the batch (pairs, frame size), the pyramid and the kernels
 themselves, so they are measured on the caller's own workload: coordinate descent, one
@@ -27,14 +29,15 @@ from typing import Callable, Dict, List, Optional, Sequence, Tuple
 
 import svpf
 
-# candidate values of each knob (the defaults are the library's: 1.0, 4096, 4096)
+# candidate values of each knob (the library's defaults are DEFAULT)
 SPACE: Dict[str, Tuple] = {
     "slot_scale": (0.5, 0.7, 1.0, 1.4, 2.0, 2.8),
     "sweep_grid": (1024, 2048, 4096, 8192),
     "coarse_slots": (1024, 2048, 4096, 8192),
+    "fine_run": (1, 2, 4, 8),
 }
-DEFAULT: Dict[str, float] = {"slot_scale": 1.0, "sweep_grid": 4096, "coarse_slots": 4096}
-ORDER = ("slot_scale", "sweep_grid", "coarse_slots")
+DEFAULT: Dict[str, float] = {"slot_scale": 1.0, "sweep_grid": 4096, "coarse_slots": 4096, "fine_run": 1}
+ORDER = ("slot_scale", "fine_run", "sweep_grid", "coarse_slots")
 
 
 @dataclass
@@ -56,7 +59,7 @@ def apply(ctx: svpf.Context, setting: Dict[str, float]) -> None:
     s = dict(DEFAULT)
     s.update(setting)
     ctx.set_tuning(slot_scale=float(s["slot_scale"]), coarse_slots=int(s["coarse_slots"]),
-                   sweep_grid=int(s["sweep_grid"]))
+                   sweep_grid=int(s["sweep_grid"]), fine_run=int(s["fine_run"]))
 
 
 def descend(measure: Callable[[Dict[str, float]], float], space: Dict[str, Sequence] = SPACE,

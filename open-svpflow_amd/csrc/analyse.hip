@@ -2085,7 +2085,7 @@ hipError_t launch_sweep_init(const Chain* d_chains, int nchains, int max_n, cons
 }
 
 hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s,
-                         int level, bool fast, bool l0, double slot_scale, int coarse_slots) {
+                         int level, bool fast, bool l0, double slot_scale, int coarse_slots, int fine_run) {
     // segments per chain: a multiple of q so that the slot count is a multiple of the 8
     // XCDs, and no segment much shorter than seg_min blocks.  Fewer slots put more items
     // of a slot in flight at once (read before their predecessor is stored); more slots
@@ -2102,7 +2102,9 @@ hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const Sw
     const double r = 32.0 / nchains;
     const double t0 = fast ? 3584.0 * std::pow(r, 0.8) : 2048.0 * std::sqrt(r);
     const int target = level == 0 ? (int)std::min(8192.0, std::max(256.0, t0 * slot_scale)) : coarse_slots;
-    const int run = level == 0 ? 1 : 8;
+    // runs at the finest level: consecutive positions searched by one wave keep the
+    // reference rows of a block's window in the caches for the next (fine_run: 1, 2, 4, 8)
+    const int run = level == 0 ? ((fast && l0) ? fine_run : 1) : 8;
     const int seg_min = fast ? 1024 : 2048;
     int g8 = 8;
     for (int a = nchains % 8; a != 0;) {
@@ -2117,8 +2119,17 @@ hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const Sw
     const int nslots = nchains * nseg;
     const long long total = (long long)nslots * (((max_n + nseg - 1) / nseg + run - 1) / run);
     const unsigned grid = (unsigned)std::min<long long>(total, 1 << 22);
-    if (fast && run == 1 && l0)
+    if (fast && l0 && run == 1)
         hipLaunchKernelGGL((sweep0_kernel<true, 1, true>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg,
+                           total);
+    else if (fast && l0 && run == 2)
+        hipLaunchKernelGGL((sweep0_kernel<true, 2, true>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg,
+                           total);
+    else if (fast && l0 && run == 4)
+        hipLaunchKernelGGL((sweep0_kernel<true, 4, true>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg,
+                           total);
+    else if (fast && l0)
+        hipLaunchKernelGGL((sweep0_kernel<true, 8, true>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg,
                            total);
     else if (fast && run == 1)
         hipLaunchKernelGGL((sweep0_kernel<true, 1>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);

@@ -32,6 +32,7 @@ struct svpf_ctx {
     // block changes (0: none, every change waits for the next sweep).
     int chase = 0;
     double slot_scale = 1.0;   // factor on the finest level's sweep-0 slot count
+    int fine_run = 1;          // finest-level positions per sweep-0 item
     int coarse_slots = 4096;   // sweep-0 slots of the coarser levels
     int sweep_grid = 4096;     // workgroups of a later sweep
     // Analyse sweep statistics per level since profiling was last enabled (device, 16 x 4,
