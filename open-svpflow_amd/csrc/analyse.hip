@@ -1,17 +1,34 @@
 // Analyse block search for gfx950 (replaces crates/svpflow1/src/analyse_search.rs).
 //
 // Execution model
-//   * One wavefront per (frame pair, direction) chain walks one pyramid level in the
-//     reference's exact visit order (analyse_search.rs:512-680).  The order is a single
-//     dependency chain (each block seeds from already-searched neighbours), so blocks
-//     are never reordered; parallelism comes from candidates and from chains.
-//   * Within a block, the candidates of one search step go across the 64 lanes: each
-//     lane computes one candidate's full cost (luma SAD/SATD + 4x chroma).  The step's
-//     winner is the lane argmin with the lowest list index on ties, which equals the
-//     reference's sequential strict-`<` running minimum (exact_check, :1122-1155): the
-//     early-outs of exact_check / exact_sad_if_better only skip candidates whose full
-//     cost cannot be below the running best (penalties are monotone for pnew >= -256;
-//     below that a lane-serial scan reproduces the skip rules literally).
+//   * A pyramid level of every (frame pair, direction) chain of the batch is searched
+//     as the fixed point of its dependency graph.  The reference visits the blocks in
+//     one order (analyse_search.rs:480-681), each block seeding from the 3x3 neighbours
+//     visited before it and, at the coarse levels, comparing with the running count of
+//     bad blocks; a block's result depends only on blocks earlier in that order, so the
+//     fixed point is unique and equals the sequential result.
+//   * Sweep 0 evaluates every block once (sweep0_kernel), dispatched in visit order over
+//     interleaved segments so that most blocks already see their predecessors' results;
+//     validate_kernel then queues every block whose recorded seeds differ from its
+//     neighbours' results.  Each later sweep (sweep_kernel) re-evaluates the queued
+//     blocks and queues the later-visited neighbours of every block whose vector
+//     changed, until a sweep changes nothing.  A block whose inputs are unchanged since
+//     its last evaluation is skipped (seed record), and a refinement that starts from
+//     the same predictor winner is reused (memo).  Coarse levels rebuild the running
+//     bad-block counts after every sweep (sweep_scan_kernel).  A level that has not
+//     converged within the sweep limit is searched sequentially (search_level_kernel).
+//   * One wavefront evaluates one block.  Its inputs (neighbours, predictors, memo,
+//     seed record, previous result) arrive in one round of loads, one word per lane.
+//     The candidates of a search step go 16 per pass, 4 lanes each: lane `part` owns a
+//     4x4 luma tile (and parts 0-1 the chroma tiles), costs are packed 16-bit SATD or
+//     v_sad_u8, and the step's winner is the lowest (cost, list index) across groups
+//     (DPP / permlane argmin), which equals the reference's sequential strict-`<`
+//     running minimum (exact_check, :1122-1155); candidates that cannot win (out of the
+//     bounds, or a motion penalty already at the best cost) are dropped before a pass.
+//   * sweep0_kernel's tile-plan instantiation (three waves per SIMD) handles the blocks
+//     whose tiles lie inside the level and passes the rest to deferred_kernel; the
+//     finest level of a multi-level search has its own instantiation of both sweep
+//     kernels, without the coarse levels' state.
 //   * Level transitions (interpolation + opposite-field projection, global predictor,
 //     score sort, child visit order) run as separate kernels over all chains, so a
 //     batch of pairs never returns to the host between levels.
