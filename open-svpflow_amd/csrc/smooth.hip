@@ -362,6 +362,15 @@ __device__ __forceinline__ int sample(const Src& s, const PP& p, int sx, int sy,
     return d[(size_t)sub * s.span + (size_t)(y >> s.shift) * s.stride + (x >> s.shift)];
 }
 
+// One thread per (plane row, tile) of one frame: the pixels of a tile row share their
+// motion samples (the 4 grid corners of every vector set), the row weights and the
+// vertical interpolations, so those are computed once; each pixel then takes its column
+// weight, its source taps and the blends.  Tiles (render_tiles, renderer.rs:1925-2010)
+// are drawn in order -1 (the origin strip), 0, 1, ...; the first tile reaching the plane
+// end takes the rest of the plane, and a later tile overwrites an earlier one, so a
+// pixel at or past the origin belongs to tile min((x - origin) / block, last).
+// Thread x-index k covers tile k - 1; a frame copied whole (decision != 2) is copied in
+// runs of 8 columns per thread instead.
 __global__ void __launch_bounds__(256) render_kernel(const FrameJob* __restrict__ jobs, const FramePlan* __restrict__ plans,
                                                      const SmoothGeom* __restrict__ gp) {
     const int plane = blockIdx.z % 3;
@@ -369,13 +378,13 @@ __global__ void __launch_bounds__(256) render_kernel(const FrameJob* __restrict_
     const SmoothGeom& g = *gp;
     const bool chroma = plane != 0;
     const int pw = chroma ? g.width / 2 : g.width, ph = chroma ? g.height / 2 : g.height;
-    const int px = blockIdx.x * blockDim.x + threadIdx.x, py = blockIdx.y;
-    if (px >= pw || py >= ph) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, py = blockIdx.y;
+    if (py >= ph) return;
     const FramePlan plan = plans[blockIdx.z / 3];
-    uint8_t* out = j.dst[plane] + (size_t)py * j.dstride[plane] + px;
+    uint8_t* orow = j.dst[plane] + (size_t)py * j.dstride[plane];
     if (plan.decision != 2) {  // copy of source n or n+1 (padded_output with no padding)
-        const uint8_t* s = plan.decision == 1 ? j.src1[plane] : j.src0[plane];
-        *out = s[(size_t)py * j.sstride[plane] + px];
+        const uint8_t* srow = (plan.decision == 1 ? j.src1[plane] : j.src0[plane]) + (size_t)py * j.sstride[plane];
+        for (int x = 8 * k; x < min(8 * k + 8, pw); x++) orow[x] = ((gm_u8*)srow)[x];
         return;
     }
     const bool interp = g.interp != 0;
@@ -396,49 +405,69 @@ __global__ void __launch_bounds__(256) render_kernel(const FrameJob* __restrict_
     mp.ox = chroma ? g.cox / 2 : g.cox;
     mp.oy = chroma ? g.coy / 2 : g.coy;
     if (p.bw <= 0 || p.bh <= 0) return;
-    // Tile of this pixel (render_tiles, renderer.rs:1925-2010): tiles are drawn in order
-    // -1 (the origin strip), 0, 1, ..., the first tile reaching the plane end takes the
-    // rest of the plane, and a later tile overwrites an earlier one, so a pixel at or past
-    // the origin always belongs to a tile >= 0.
     const int mw = g.mw, mh = g.mh;
-    auto tile_of = [](int v, int o, int b, int size, int n, int& base) -> int {
-        if (v < o) { base = 0; return -1; }
-        // (v - o) >= 0: a shift when the block size is a power of two
-        int t = (b & (b - 1)) == 0 ? (v - o) >> __builtin_ctz((unsigned)b) : (v - o) / b;
-        const int ext = max((size - o + b - 1) / b - 1, 0);  // first tile reaching the plane end
-        t = min(t, min(n - 1, ext));
-        base = o + b * t;
-        return t;
-    };
-    int bx, by;
-    const int gx = tile_of(px, p.ox, p.bw, pw, mw, bx), gy = tile_of(py, p.oy, p.bh, ph, mh, by);
+    // this thread's columns [xb, xe) and tile gx with base bx
+    const int gx = k - 1;
+    int xb, xe, bx;
+    if (gx < 0) {
+        xb = 0; xe = min(p.ox, pw); bx = 0;
+    } else {
+        if (p.ox >= pw) return;
+        const int ext = max((pw - p.ox + p.bw - 1) / p.bw - 1, 0);  // first tile reaching the plane end
+        const int last = min(mw - 1, ext);
+        if (gx > last) return;
+        bx = p.ox + p.bw * gx;
+        xb = bx;
+        xe = gx == last ? pw : min(bx + p.bw, pw);
+    }
+    if (xb >= xe) return;
+    int gy, by;
+    if (py < p.oy) {
+        gy = -1; by = 0;
+    } else {
+        const int ext = max((ph - p.oy + p.bh - 1) / p.bh - 1, 0);
+        gy = min((py - p.oy) / p.bh, min(mh - 1, ext));
+        by = p.oy + p.bh * gy;
+    }
     const int x0 = max(gx, 0), y0 = max(gy, 0);
     const int nx = (gx >= 0 && gx != mw - 1) ? 1 : 0, ny = (gy >= 0 && gy != mh - 1) ? mw : 0;
     const int i00 = y0 * mw + x0, i01 = i00 + nx, i10 = i00 + ny, i11 = i00 + ny + nx;
-    const int lx = px - bx, ly = py - by;
-    const int* wx = p.xw[min(lx, 31)];
-    const int* wy = p.yw[min(ly, 31)];
+    const int* wy = p.yw[min(py - by, 31)];
     const int mc = mw * mh;
     const int t = plan.phase, inv = 256 - plan.phase;
-    // motion vector of one motion set at this pixel (motion_samples + interpolate_*)
-    auto motion = [&](const uint16_t* mx, const uint16_t* my, int tt, int& dx, int& dy) {
+    // a vector set at this tile row (motion_samples + the vertical interpolation);
+    // its value at a pixel is mv_at (interpolate_* with the column weight)
+    struct MV {
+        int rx0, rx1, ry0, ry1;
+    };
+    auto mv_row = [&](const uint16_t* mx, const uint16_t* my, int tt) -> MV {
         const bool hx = p.xdiv == 2, hy = p.ydiv == 2;
         const int a0 = div12(mval(mx, i00, mc, tt), hx), b0 = div12(mval(my, i00, mc, tt), hy);
-        if (!interp) { dx = a0; dy = b0; return; }
+        if (!interp) return MV{a0, a0, b0, b0};
         const int a1 = div12(mval(mx, i01, mc, tt), hx), a2 = div12(mval(mx, i10, mc, tt), hx);
         const int a3 = div12(mval(mx, i11, mc, tt), hx);
         const int b1 = div12(mval(my, i01, mc, tt), hy), b2 = div12(mval(my, i10, mc, tt), hy);
         const int b3 = div12(mval(my, i11, mc, tt), hy);
-        const int rx0 = interp_pair(a0, a2, p.bh, p.ys, wy), rx1 = interp_pair(a1, a3, p.bh, p.ys, wy);
-        const int ry0 = interp_pair(b0, b2, p.bh, p.ys, wy), ry1 = interp_pair(b1, b3, p.bh, p.ys, wy);
-        dx = interp_pair(rx0, rx1, p.bw, p.xs, wx);
-        dy = interp_pair(ry0, ry1, p.bw, p.xs, wx);
+        return MV{interp_pair(a0, a2, p.bh, p.ys, wy), interp_pair(a1, a3, p.bh, p.ys, wy),
+                  interp_pair(b0, b2, p.bh, p.ys, wy), interp_pair(b1, b3, p.bh, p.ys, wy)};
     };
-    auto alpha = [&](const uint8_t* m) -> int {
-        if (zero) return alpha_abs(m, mc, mp, mw, px, py);
+    auto mv_at = [&](const MV& m, const int* wx, int& dx, int& dy) {
+        if (!interp) { dx = m.rx0; dy = m.ry0; return; }
+        dx = interp_pair(m.rx0, m.rx1, p.bw, p.xs, wx);
+        dy = interp_pair(m.ry0, m.ry1, p.bw, p.xs, wx);
+    };
+    // a mask at this tile row (alpha: interpolated, or alpha_at_absolute for zero-origin)
+    struct AR {
+        int r0, r1;
+    };
+    auto alpha_row = [&](const uint8_t* m) -> AR {
+        if (zero) return AR{0, 0};
         auto at = [&](int i) { return i < mc ? (int)((gm_u8*)m)[i] : 0; };
-        const int r0 = interp_pair(at(i00), at(i10), p.bh, p.ys, wy), r1 = interp_pair(at(i01), at(i11), p.bh, p.ys, wy);
-        return byte_of(interp_pair(r0, r1, p.bw, p.xs, wx));
+        return AR{interp_pair(at(i00), at(i10), p.bh, p.ys, wy), interp_pair(at(i01), at(i11), p.bh, p.ys, wy)};
+    };
+    auto alpha_at = [&](const uint8_t* m, const AR& r, const int* wx, int px) -> int {
+        if (zero) return alpha_abs(m, mc, mp, mw, px, py);
+        return byte_of(interp_pair(r.r0, r.r1, p.bw, p.xs, wx));
     };
     Src s0, s1;
     if (g.use_super) {
@@ -448,59 +477,83 @@ __global__ void __launch_bounds__(256) render_kernel(const FrameJob* __restrict_
         s0 = Src{j.src0[plane], j.sstride[plane], 0, 0};
         s1 = Src{j.src1[plane], j.sstride[plane], 0, 0};
     }
-    const int sx = px * p.step, sy = py * p.step;
-    int v;
+    const int sy = py * p.step;
     const int algo = plan.algo;
     if (algo == 1 || algo == 2) {  // render_mode1_or_2 -> render_selected_warp (renderer.rs:308-327, 881-982)
         const bool m1 = algo == 1;
         const Src& s = m1 ? s1 : s0;
-        int dx, dy;
-        if (m1) motion(j.mv_n[2], j.mv_n[3], inv, dx, dy);
-        else motion(j.mv_n[0], j.mv_n[1], t, dx, dy);
-        const int warped = sample(s, p, sx, sy, dx, dy);
-        v = warped;
-        if (g.mask_area) {
-            const int base = sample(s, p, sx, sy, 0, 0);
-            v = blend255(warped, base, alpha(m1 ? j.mag[0] : j.mag[1]));
+        const MV m = m1 ? mv_row(j.mv_n[2], j.mv_n[3], inv) : mv_row(j.mv_n[0], j.mv_n[1], t);
+        const uint8_t* mk = m1 ? j.mag[0] : j.mag[1];
+        const AR ar = g.mask_area ? alpha_row(mk) : AR{0, 0};
+        for (int px = xb; px < xe; px++) {
+            const int* wx = p.xw[min(px - bx, 31)];
+            const int sx = px * p.step;
+            int dx, dy;
+            mv_at(m, wx, dx, dy);
+            const int warped = sample(s, p, sx, sy, dx, dy);
+            int v = warped;
+            if (g.mask_area) v = blend255(warped, sample(s, p, sx, sy, 0, 0), alpha_at(mk, ar, wx, px));
+            orow[px] = (uint8_t)v;
         }
     } else if (algo == 11 || algo == 13) {  // render_mode11_or_13 (renderer.rs:330-437, 2918-2942)
-        int dx0, dy0, dx1, dy1;
-        motion(j.mv_n[0], j.mv_n[1], t, dx0, dy0);
-        motion(j.mv_n[2], j.mv_n[3], inv, dx1, dy1);
-        const int c0 = sample(s0, p, sx, sy, 0, 0), c1 = sample(s1, p, sx, sy, 0, 0);
-        const int v0 = sample(s0, p, sx, sy, dx0, dy0), v1 = sample(s1, p, sx, sy, dx1, dy1);
-        const int mo = algo == 13 ? clamp_between(blend256(c0, c1, t), v0, v1) : blend256(v0, v1, t);
-        v = g.mask_area ? blend255(mo, blend256(c0, c1, t), alpha(j.maxmag)) : mo;
+        const MV m0 = mv_row(j.mv_n[0], j.mv_n[1], t), m1 = mv_row(j.mv_n[2], j.mv_n[3], inv);
+        const AR am = g.mask_area ? alpha_row(j.maxmag) : AR{0, 0};
+        for (int px = xb; px < xe; px++) {
+            const int* wx = p.xw[min(px - bx, 31)];
+            const int sx = px * p.step;
+            int dx0, dy0, dx1, dy1;
+            mv_at(m0, wx, dx0, dy0);
+            mv_at(m1, wx, dx1, dy1);
+            const int c0 = sample(s0, p, sx, sy, 0, 0), c1 = sample(s1, p, sx, sy, 0, 0);
+            const int v0 = sample(s0, p, sx, sy, dx0, dy0), v1 = sample(s1, p, sx, sy, dx1, dy1);
+            const int mo = algo == 13 ? clamp_between(blend256(c0, c1, t), v0, v1) : blend256(v0, v1, t);
+            orow[px] = (uint8_t)(g.mask_area ? blend255(mo, blend256(c0, c1, t), alpha_at(j.maxmag, am, wx, px)) : mo);
+        }
     } else if ((algo == 21 || algo == 22) || (algo == 23 && !plan.mode23_ready)) {  // renderer.rs:617-641, 1830-1848
         const bool m21 = algo != 22;
-        int dx0, dy0, dx1, dy1;
-        motion(j.mv_n[0], j.mv_n[1], t, dx0, dy0);
-        motion(j.mv_n[2], j.mv_n[3], inv, dx1, dy1);
-        const int c0 = sample(s0, p, sx, sy, 0, 0), c1 = sample(s1, p, sx, sy, 0, 0);
-        const int v0 = sample(s0, p, sx, sy, dx0, dy0), v1 = sample(s1, p, sx, sy, dx1, dy1);
-        const int a0 = alpha(j.cov[0]), a1 = alpha(j.cov[1]);
-        const int a = blend255(v0, v1, a0), b = blend255(v1, v0, a1);
-        const int mo = m21 ? blend256(a, b, t) : clamp_between(blend256(c0, c1, t), a, b);
-        v = g.mask_area ? blend255(mo, blend256(c0, c1, plan.limit), alpha(j.maxmag)) : mo;
-    } else {  // algo 23 (renderer.rs:1294-1537, 1850-1868)
-        int d[4][2];
-        motion(j.mv_n[0], j.mv_n[1], t, d[0][0], d[0][1]);
-        motion(j.mv_n[2], j.mv_n[3], inv, d[1][0], d[1][1]);
-        motion(j.mv2[0], j.mv2[1], t, d[2][0], d[2][1]);
-        motion(j.mv3[0], j.mv3[1], inv, d[3][0], d[3][1]);
-        const int A = sample(s0, p, sx, sy, d[0][0], d[0][1]), B = sample(s1, p, sx, sy, d[1][0], d[1][1]);
-        const int Cc = sample(s0, p, sx, sy, d[2][0], d[2][1]), D = sample(s1, p, sx, sy, d[3][0], d[3][1]);
-        const int t0 = alpha(j.cov[0]), t1 = alpha(j.cov[1]);
-        const int i0 = blend255(A, clamp_between(D, A, B), t0);
-        const int i1 = blend255(B, clamp_between(Cc, A, B), t1);
-        int mo = blend256(i0, i1, t);
-        if (g.mask_area) {
-            const int ba = sample(s0, p, sx, sy, 0, 0), bb = sample(s1, p, sx, sy, 0, 0);
-            mo = blend255(mo, blend256(ba, bb, t), alpha(j.maxmag));
+        const MV m0 = mv_row(j.mv_n[0], j.mv_n[1], t), m1 = mv_row(j.mv_n[2], j.mv_n[3], inv);
+        const AR a0r = alpha_row(j.cov[0]), a1r = alpha_row(j.cov[1]);
+        const AR am = g.mask_area ? alpha_row(j.maxmag) : AR{0, 0};
+        for (int px = xb; px < xe; px++) {
+            const int* wx = p.xw[min(px - bx, 31)];
+            const int sx = px * p.step;
+            int dx0, dy0, dx1, dy1;
+            mv_at(m0, wx, dx0, dy0);
+            mv_at(m1, wx, dx1, dy1);
+            const int c0 = sample(s0, p, sx, sy, 0, 0), c1 = sample(s1, p, sx, sy, 0, 0);
+            const int v0 = sample(s0, p, sx, sy, dx0, dy0), v1 = sample(s1, p, sx, sy, dx1, dy1);
+            const int a0 = alpha_at(j.cov[0], a0r, wx, px), a1 = alpha_at(j.cov[1], a1r, wx, px);
+            const int a = blend255(v0, v1, a0), b = blend255(v1, v0, a1);
+            const int mo = m21 ? blend256(a, b, t) : clamp_between(blend256(c0, c1, t), a, b);
+            orow[px] = (uint8_t)(g.mask_area ? blend255(mo, blend256(c0, c1, plan.limit), alpha_at(j.maxmag, am, wx, px))
+                                             : mo);
         }
-        v = mo;
+    } else {  // algo 23 (renderer.rs:1294-1537, 1850-1868)
+        const MV m0 = mv_row(j.mv_n[0], j.mv_n[1], t), m1 = mv_row(j.mv_n[2], j.mv_n[3], inv);
+        const MV m2 = mv_row(j.mv2[0], j.mv2[1], t), m3 = mv_row(j.mv3[0], j.mv3[1], inv);
+        const AR a0r = alpha_row(j.cov[0]), a1r = alpha_row(j.cov[1]);
+        const AR am = g.mask_area ? alpha_row(j.maxmag) : AR{0, 0};
+        for (int px = xb; px < xe; px++) {
+            const int* wx = p.xw[min(px - bx, 31)];
+            const int sx = px * p.step;
+            int d[4][2];
+            mv_at(m0, wx, d[0][0], d[0][1]);
+            mv_at(m1, wx, d[1][0], d[1][1]);
+            mv_at(m2, wx, d[2][0], d[2][1]);
+            mv_at(m3, wx, d[3][0], d[3][1]);
+            const int A = sample(s0, p, sx, sy, d[0][0], d[0][1]), B = sample(s1, p, sx, sy, d[1][0], d[1][1]);
+            const int Cc = sample(s0, p, sx, sy, d[2][0], d[2][1]), D = sample(s1, p, sx, sy, d[3][0], d[3][1]);
+            const int t0 = alpha_at(j.cov[0], a0r, wx, px), t1 = alpha_at(j.cov[1], a1r, wx, px);
+            const int i0 = blend255(A, clamp_between(D, A, B), t0);
+            const int i1 = blend255(B, clamp_between(Cc, A, B), t1);
+            int mo = blend256(i0, i1, t);
+            if (g.mask_area) {
+                const int ba = sample(s0, p, sx, sy, 0, 0), bb = sample(s1, p, sx, sy, 0, 0);
+                mo = blend255(mo, blend256(ba, bb, t), alpha_at(j.maxmag, am, wx, px));
+            }
+            orow[px] = (uint8_t)mo;
+        }
     }
-    *out = (uint8_t)v;
 }
 
 }  // namespace
@@ -694,8 +747,11 @@ hipError_t launch_smooth_masks(const FrameJob* d_jobs, FramePlan* d_plans, int n
 }
 
 hipError_t launch_smooth_render(const FrameJob* d_jobs, const FramePlan* d_plans, int njobs, const SmoothGeom* d_geom,
-                                int width, int height, hipStream_t s) {
-    dim3 grid((width + 255) / 256, height, 3 * njobs);
+                                int width, int height, int mw, hipStream_t s) {
+    // threads per row: the origin strip and the tiles (at most mw of them), or the runs
+    // of 8 columns of a copied frame
+    const int per_row = std::max(mw + 1, (width + 7) / 8);
+    dim3 grid((per_row + 255) / 256, height, 3 * njobs);
     hipLaunchKernelGGL(render_kernel, grid, dim3(256), 0, s, d_jobs, d_plans, d_geom);
     return hipGetLastError();
 }

@@ -18,7 +18,7 @@ hipError_t launch_smooth_prep(const PrepJob* d_jobs, int njobs, const SmoothGeom
 hipError_t launch_smooth_masks(const FrameJob* d_jobs, FramePlan* d_plans, int njobs, const SmoothGeom* d_geom, int count,
                                int mw, int mh, bool coverage, hipStream_t s);
 hipError_t launch_smooth_render(const FrameJob* d_jobs, const FramePlan* d_plans, int njobs, const SmoothGeom* d_geom,
-                                int width, int height, hipStream_t s);
+                                int width, int height, int mw, hipStream_t s);
 hipError_t launch_smooth_render_float(const FrameJob* d_jobs, const FramePlan* d_plans, int njobs,
                                       const SmoothGeom* d_geom, int width, int height, hipStream_t s);
 
@@ -648,7 +648,7 @@ int svpf_smooth_render(svpf_smooth* s, int32_t k0, int32_t count, const svpf_fra
         SVPF_HIP(launch_smooth_render_float((const FrameJob*)d_up, d_plans, count, s->d_geom, s->width, s->height,
                                             s->ctx->stream));
     else
-        SVPF_HIP(launch_smooth_render((const FrameJob*)d_up, d_plans, count, s->d_geom, s->width, s->height,
+        SVPF_HIP(launch_smooth_render((const FrameJob*)d_up, d_plans, count, s->d_geom, s->width, s->height, s->mw,
                                       s->ctx->stream));
     prof_end(s->ctx, ph);
     if (h_decision) {  // (decision, algo) per frame; synchronizes
