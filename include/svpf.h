@@ -161,16 +161,20 @@ int svpf_analyse_set_max_sweeps(svpf_ctx* ctx, int32_t max_sweeps);
  *                changes (default 0: none; measured slower on large batches, whose
  *                early sweeps then wait on their longest chase);
  *   slot_scale   factor on the finest level's sweep-0 slot count (default 1.0);
- *   coarse_slots sweep-0 slots of the coarser levels (default 4096);
+ *   coarse_slots sweep-0 slots of the coarser levels (default 256: one segment per chain
+ *                from 256 chains on; their visit order is sorted by parent score, so a
+ *                block's earlier neighbours lie anywhere before it);
  *   sweep_grid   workgroups of a later sweep (default 4096);
  *   fine_run     consecutive visit positions a finest-level sweep-0 item searches in
- *                order (1, 2, 4 or 8; default 1). */
+ *                order (1, 2, 4 or 8; default 1);
+ *   coarse_run   the same at the coarser levels (4, 8, 16 or 32; default 16). */
 typedef struct svpf_analyse_tuning {
     int32_t chase;
     double slot_scale;
     int32_t coarse_slots;
     int32_t sweep_grid;
     int32_t fine_run;
+    int32_t coarse_run;
 } svpf_analyse_tuning;
 int svpf_analyse_set_tuning(svpf_ctx* ctx, const svpf_analyse_tuning* t);
 

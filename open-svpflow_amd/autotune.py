@@ -7,7 +7,8 @@ settings only decide how fast it gets there:
   more blocks at a segment start (evaluated before the segment above reaches them),
   fewer slots put more items of one slot in flight at once (evaluated before their
   predecessor is stored); both cost re-evaluations in the later sweeps.
-* ``coarse_slots`` -- sweep-0 slots of the coarser levels.
+* ``coarse_slots`` -- sweep-0 slots of the coarser levels, and ``coarse_run`` the
+  positions of their visit order one item searches in order.
 * ``sweep_grid``   -- workgroups of a later sweep (a persistent grid over the queue).
 * ``fine_run``     -- consecutive finest-level positions one sweep-0 item searches in
   order (a block's reference rows stay cached for the next).
@@ -33,11 +34,12 @@ import svpf
 SPACE: Dict[str, Tuple] = {
     "slot_scale": (0.5, 0.7, 1.0, 1.4, 2.0, 2.8),
     "sweep_grid": (1024, 2048, 4096, 8192),
-    "coarse_slots": (1024, 2048, 4096, 8192),
+    "coarse_slots": (256, 512, 1024, 4096),
     "fine_run": (1, 2, 4, 8),
+    "coarse_run": (4, 8, 16, 32),
 }
-DEFAULT: Dict[str, float] = {"slot_scale": 1.0, "sweep_grid": 4096, "coarse_slots": 4096, "fine_run": 1}
-ORDER = ("slot_scale", "fine_run", "sweep_grid", "coarse_slots")
+DEFAULT: Dict[str, float] = {"slot_scale": 1.0, "sweep_grid": 4096, "coarse_slots": 256, "fine_run": 1, "coarse_run": 16}
+ORDER = ("slot_scale", "fine_run", "sweep_grid", "coarse_slots", "coarse_run")
 
 
 @dataclass
@@ -59,7 +61,7 @@ def apply(ctx: svpf.Context, setting: Dict[str, float]) -> None:
     s = dict(DEFAULT)
     s.update(setting)
     ctx.set_tuning(slot_scale=float(s["slot_scale"]), coarse_slots=int(s["coarse_slots"]),
-                   sweep_grid=int(s["sweep_grid"]), fine_run=int(s["fine_run"]))
+                   sweep_grid=int(s["sweep_grid"]), fine_run=int(s["fine_run"]), coarse_run=int(s["coarse_run"]))
 
 
 def descend(measure: Callable[[Dict[str, float]], float], space: Dict[str, Sequence] = SPACE,

@@ -2102,7 +2102,8 @@ hipError_t launch_sweep_init(const Chain* d_chains, int nchains, int max_n, cons
 }
 
 hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s,
-                         int level, bool fast, bool l0, double slot_scale, int coarse_slots, int fine_run) {
+                         int level, bool fast, bool l0, double slot_scale, int coarse_slots, int fine_run,
+                         int coarse_run) {
     // segments per chain: a multiple of q so that the slot count is a multiple of the 8
     // XCDs, and no segment much shorter than seg_min blocks.  Fewer slots put more items
     // of a slot in flight at once (read before their predecessor is stored); more slots
@@ -2113,15 +2114,19 @@ hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const Sw
     // 2048 * sqrt(32 / chains), rounded down to whole segments per chain.  Three (the
     // tile-plan kernel): about 3600 slots at 32 chains and again 720 at 240 (3
     // segments beat 2 and 4), i.e. about 3584 * (32 / chains)^0.8, with segments of
-    // 1024 blocks or more.  Coarser levels take items of 8 positions (kRun) and about
-    // 4096 slots (runs of 4 with 2048 slots: 3% slower at 120 pairs; runs of 16: no
-    // faster there, and twice the latency of a small batch's sweep 0).
+    // 1024 blocks or more (the finest level's factor slot_scale and run fine_run are
+    // tuned per workload).  The coarser levels' orders are sorted by parent score, so a
+    // block's earlier-visited neighbours lie anywhere before it: one segment per chain
+    // (coarse_slots 256 at 240 chains) keeps in flight only the positions just before
+    // the ones being searched (1080p, 120 pairs: 6% faster than 4096 slots), and items of
+    // coarse_run positions (16: four parents' children; 2% faster than 8 there) search
+    // siblings in order.
     const double r = 32.0 / nchains;
     const double t0 = fast ? 3584.0 * std::pow(r, 0.8) : 2048.0 * std::sqrt(r);
     const int target = level == 0 ? (int)std::min(8192.0, std::max(256.0, t0 * slot_scale)) : coarse_slots;
     // runs at the finest level: consecutive positions searched by one wave keep the
     // reference rows of a block's window in the caches for the next (fine_run: 1, 2, 4, 8)
-    const int run = level == 0 ? ((fast && l0) ? fine_run : 1) : 8;
+    const int run = level == 0 ? ((fast && l0) ? fine_run : 1) : (fast ? coarse_run : 8);
     const int seg_min = fast ? 1024 : 2048;
     int g8 = 8;
     for (int a = nchains % 8; a != 0;) {
@@ -2150,6 +2155,12 @@ hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const Sw
                            total);
     else if (fast && run == 1)
         hipLaunchKernelGGL((sweep0_kernel<true, 1>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
+    else if (fast && run == 4)
+        hipLaunchKernelGGL((sweep0_kernel<true, 4>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
+    else if (fast && run == 16)
+        hipLaunchKernelGGL((sweep0_kernel<true, 16>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
+    else if (fast && run == 32)
+        hipLaunchKernelGGL((sweep0_kernel<true, 32>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
     else if (fast)
         hipLaunchKernelGGL((sweep0_kernel<true, 8>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
     else if (run == 1)

@@ -13,7 +13,8 @@ namespace svpf {
 hipError_t launch_search_level(const Chain* d_chains, int nchains, hipStream_t s);
 hipError_t launch_sweep_init(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s);
 hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s,
-                         int level, bool fast, bool l0, double slot_scale, int coarse_slots, int fine_run);
+                         int level, bool fast, bool l0, double slot_scale, int coarse_slots, int fine_run,
+                         int coarse_run);
 hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s, bool l0);
 hipError_t launch_sweep_stats(const SweepState& st, int sweeps, long long total_blocks, unsigned long long* out4,
                               hipStream_t s);
@@ -176,7 +177,7 @@ int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total
     if (!no_alloc) h_count = ctx->h_count;
     const int grid = (int)std::min<size_t>(std::max<size_t>(total_blocks, 1), (size_t)ctx->sweep_grid);
     SVPF_HIP(launch_sweep0(dch, nch, max_n, st, ctx->stream, level, fast, l0, ctx->slot_scale, ctx->coarse_slots,
-                           ctx->fine_run));
+                           ctx->fine_run, ctx->coarse_run));
     if (fast) SVPF_HIP(launch_deferred(dch, grid, st, 0, ctx->stream));
     if (scan) SVPF_HIP(launch_sweep_scan(dch, nch, st, 0, ctx->stream));
     SVPF_HIP(launch_validate(dch, nch, max_n, st, ctx->stream));
@@ -885,13 +886,16 @@ int svpf_analyse_set_tuning(svpf_ctx* ctx, const svpf_analyse_tuning* t) {
     if (!ctx || !t) return fail(SVPF_ERR_ARG, "svpf_analyse_set_tuning: null argument");
     if (t->fine_run != 0 && t->fine_run != 1 && t->fine_run != 2 && t->fine_run != 4 && t->fine_run != 8)
         return fail(SVPF_ERR_ARG, "svpf_analyse_set_tuning: fine_run must be 1, 2, 4 or 8");
+    if (t->coarse_run != 0 && t->coarse_run != 4 && t->coarse_run != 8 && t->coarse_run != 16 && t->coarse_run != 32)
+        return fail(SVPF_ERR_ARG, "svpf_analyse_set_tuning: coarse_run must be 4, 8, 16 or 32");
     if (t->chase > 65536 || t->slot_scale < 0.0 || t->slot_scale > 16.0 || t->coarse_slots < 0 ||
         t->coarse_slots > (1 << 20) || t->sweep_grid < 0 || t->sweep_grid > (1 << 20))
         return fail(SVPF_ERR_ARG, "svpf_analyse_set_tuning: setting out of range");
     ctx->chase = std::max(t->chase, 0);
     ctx->slot_scale = t->slot_scale == 0.0 ? 1.0 : t->slot_scale;
     ctx->fine_run = t->fine_run == 0 ? 1 : t->fine_run;
-    ctx->coarse_slots = t->coarse_slots == 0 ? 4096 : t->coarse_slots;
+    ctx->coarse_slots = t->coarse_slots == 0 ? 256 : t->coarse_slots;
+    ctx->coarse_run = t->coarse_run == 0 ? 16 : t->coarse_run;
     ctx->sweep_grid = t->sweep_grid == 0 ? 4096 : t->sweep_grid;
     return SVPF_OK;
 }

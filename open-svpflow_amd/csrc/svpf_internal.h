@@ -33,7 +33,8 @@ struct svpf_ctx {
     int chase = 0;
     double slot_scale = 1.0;   // factor on the finest level's sweep-0 slot count
     int fine_run = 1;          // finest-level positions per sweep-0 item
-    int coarse_slots = 4096;   // sweep-0 slots of the coarser levels
+    int coarse_slots = 256;    // sweep-0 slots of the coarser levels
+    int coarse_run = 16;       // coarser-level positions per sweep-0 item
     int sweep_grid = 4096;     // workgroups of a later sweep
     // Analyse sweep statistics per level since profiling was last enabled (device, 16 x 4,
     // added by a kernel so that profiling adds no host round trip):
