@@ -25,7 +25,7 @@ def _surface(best):
 
 
 def test_descend_finds_separable_minimum():
-    best = {"slot_scale": 1.4, "sweep_grid": 2048, "coarse_slots": 8192}
+    best = {"slot_scale": 1.4, "sweep_grid": 2048, "coarse_slots": 1024, "fine_run": 8, "coarse_run": 4}
     measure, calls = _surface(best)
     got, cost, trials = autotune.descend(measure)
     assert got == best and cost == 100.0
@@ -45,7 +45,8 @@ def test_descend_stops_when_nothing_improves():
 
 
 def test_descend_respects_deadline():
-    measure, calls = _surface({"slot_scale": 2.8, "sweep_grid": 1024, "coarse_slots": 1024})
+    measure, calls = _surface({"slot_scale": 2.8, "sweep_grid": 1024, "coarse_slots": 1024, "fine_run": 2,
+                                "coarse_run": 32})
     got, cost, trials = autotune.descend(measure, deadline=time.monotonic() - 1.0)
     assert len(trials) == 1 and got == autotune.DEFAULT
 
