@@ -1479,9 +1479,11 @@ __global__ void __launch_bounds__(kScanThreads) sweep_scan_kernel(const Chain* _
 // Returns whether the block's stored vector changed; *me_out = its visit position.
 template <bool kFast, bool kL0 = false>
 __device__ __forceinline__ bool sweep_item(const Chain* __restrict__ chains, int ci, int idx, const SweepState& st,
-                                           int s, int* me_out = nullptr) {
+                                           int s, int* me_out = nullptr, int clear_parity = -1) {
     const Chain ch = uniform_load(&chains[ci]);
     const LevelGeom& g = ch.geom;
+    // the block leaves list clear_parity (it may be queued there again from sweep s + 1 on)
+    if (clear_parity >= 0 && lane_id() == 0) st.flags[clear_parity][ch.base + idx] = 0;
     Blk b;
     init_blk(ch, g, b);
     bool bad;
@@ -1660,11 +1662,16 @@ template <bool kL0>
 __global__ void __launch_bounds__(kWave, 2) sweep_kernel(const Chain* __restrict__ chains, SweepState st, int s) {
     const uint32_t n = (uint32_t)uniform_int((int)__atomic_load_n(&st.counts[s], __ATOMIC_RELAXED));
     const int p = s & 1;
+    const int chase = st.chase;
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
         const unsigned long long item = st.wl[p][i];
         // uniform across the wave: keeps the chain's descriptors in scalar registers
         const int ci = __builtin_amdgcn_readfirstlane((int)(item >> 32));
         int idx = __builtin_amdgcn_readfirstlane((int)(item & 0xFFFFFFFFu));
+        if (chase == 0) {  // every queued block once (its flag), no claim needed
+            sweep_item<false, kL0>(chains, ci, idx, st, s, nullptr, p);
+            continue;
+        }
         const uint32_t base = (uint32_t)uniform_int((int)chains[ci].base);
         if (lane_id() == 0) st.flags[p][base + idx] = 0;
         if (!claim_block(st, base, idx, s)) {  // a chase of this sweep evaluated it
@@ -1673,7 +1680,7 @@ __global__ void __launch_bounds__(kWave, 2) sweep_kernel(const Chain* __restrict
         }
         for (int k = 0;; k++) {
             int me = 0;
-            if (!sweep_item<false, kL0>(chains, ci, idx, st, s, &me) || k >= st.chase) break;
+            if (!sweep_item<false, kL0>(chains, ci, idx, st, s, &me) || k >= chase) break;
             const int next = order_successor(chains, ci, idx, me);
             if (next < 0 || !claim_block(st, base, next, s)) break;
             // this wave's store of idx before the successor reads it
