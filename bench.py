@@ -340,7 +340,7 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=64, help="frame pairs of the CPU baseline sample (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=15.0,
                     help="length of the config-5 serving run that gives the p50/p99 frame latency (0: skip)")
-    ap.add_argument("--serve-budget", type=float, default=0.5,
+    ap.add_argument("--serve-budget", type=float, default=0.4,
                     help="serving admission: fraction of the batched source pixel rate the admitted streams may deliver")
     ap.add_argument("--config4-clips", type=int, default=64, help="clips of the config-4 batched run (0: skip)")
     ap.add_argument("--config4-pairs-per-call", type=int, default=480,
@@ -504,7 +504,7 @@ def main():
 
     serving = None
     if args.serve_seconds > 0:
-        # throughput budget: half the source pixel rate the batched step sustained (serving
+        # throughput budget: 40% of the source pixel rate the batched step sustained (serving
         # steps are small batches, whose level searches are bound by latency, not
         # throughput; at 75% the admitted load outruns them and latency grows without bound)
         src_rate = clip.n * clip.w * clip.h * args.steps / wall
