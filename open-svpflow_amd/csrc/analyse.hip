@@ -256,6 +256,8 @@ __device__ __forceinline__ void group_cost(const Blk& b, int mvx, int mvy, bool 
 // ---- compile-time tile path of 4x4 and 8x8 blocks
 
 typedef __attribute__((address_space(1))) const uint8_t g_u8;
+// Chain descriptors are written by the host before a launch and only read by kernels.
+typedef __attribute__((address_space(4))) const struct Chain const_chain;
 typedef __attribute__((address_space(1))) const uint32_t __attribute__((aligned(1))) g_u32u;
 __device__ __forceinline__ uint32_t gld32(const g_u8* p) { return *reinterpret_cast<const g_u32u*>(p); }
 
@@ -1480,7 +1482,9 @@ __global__ void __launch_bounds__(kScanThreads) sweep_scan_kernel(const Chain* _
 template <bool kFast, bool kL0 = false>
 __device__ __forceinline__ bool sweep_item(const Chain* __restrict__ chains, int ci, int idx, const SweepState& st,
                                            int s, int* me_out = nullptr, int clear_parity = -1) {
-    const Chain ch = uniform_load(&chains[ci]);
+    // the descriptor read in place through the constant address space: scalar loads the
+    // compiler may repeat instead of keeping (and spilling) every field
+    const Chain& ch = *(const Chain*)((const_chain*)chains + ci);
     const LevelGeom& g = ch.geom;
     // the block leaves list clear_parity (it may be queued there again from sweep s + 1 on)
     if (clear_parity >= 0 && lane_id() == 0) st.flags[clear_parity][ch.base + idx] = 0;
