@@ -166,7 +166,7 @@ int svpf_analyse_set_max_sweeps(svpf_ctx* ctx, int32_t max_sweeps);
  *                block's earlier neighbours lie anywhere before it);
  *   sweep_grid   workgroups of a later sweep (default 4096);
  *   fine_run     consecutive visit positions a finest-level sweep-0 item searches in
- *                order (1, 2, 4 or 8; default 1);
+ *                order (1, 2, 4, 8, 16 or 32; default 1);
  *   coarse_run   the same at the coarser levels (4, 8, 16 or 32; default 16). */
 typedef struct svpf_analyse_tuning {
     int32_t chase;

@@ -35,7 +35,7 @@ SPACE: Dict[str, Tuple] = {
     "slot_scale": (0.5, 0.7, 1.0, 1.4, 2.0, 2.8),
     "sweep_grid": (1024, 2048, 4096, 8192),
     "coarse_slots": (256, 512, 1024, 4096),
-    "fine_run": (1, 2, 4, 8),
+    "fine_run": (1, 2, 4, 8, 16, 32),
     "coarse_run": (4, 8, 16, 32),
 }
 DEFAULT: Dict[str, float] = {"slot_scale": 1.0, "sweep_grid": 4096, "coarse_slots": 256, "fine_run": 1, "coarse_run": 16}

@@ -2136,7 +2136,7 @@ hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const Sw
     const double t0 = fast ? 3584.0 * std::pow(r, 0.8) : 2048.0 * std::sqrt(r);
     const int target = level == 0 ? (int)std::min(8192.0, std::max(256.0, t0 * slot_scale)) : coarse_slots;
     // runs at the finest level: consecutive positions searched by one wave keep the
-    // reference rows of a block's window in the caches for the next (fine_run: 1, 2, 4, 8)
+    // reference rows of a block's window in the caches for the next (fine_run: 1 to 32)
     const int run = level == 0 ? ((fast && l0) ? fine_run : 1) : (fast ? coarse_run : 8);
     const int seg_min = fast ? 1024 : 2048;
     int g8 = 8;
@@ -2160,6 +2160,12 @@ hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const Sw
                            total);
     else if (fast && l0 && run == 4)
         hipLaunchKernelGGL((sweep0_kernel<true, 4, true>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg,
+                           total);
+    else if (fast && l0 && run == 16)
+        hipLaunchKernelGGL((sweep0_kernel<true, 16, true>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg,
+                           total);
+    else if (fast && l0 && run == 32)
+        hipLaunchKernelGGL((sweep0_kernel<true, 32, true>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg,
                            total);
     else if (fast && l0)
         hipLaunchKernelGGL((sweep0_kernel<true, 8, true>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg,

@@ -884,8 +884,9 @@ int svpf_analyse_pair(svpf_ctx* ctx, const svpf_analyse_params* p, const svpf_fr
 
 int svpf_analyse_set_tuning(svpf_ctx* ctx, const svpf_analyse_tuning* t) {
     if (!ctx || !t) return fail(SVPF_ERR_ARG, "svpf_analyse_set_tuning: null argument");
-    if (t->fine_run != 0 && t->fine_run != 1 && t->fine_run != 2 && t->fine_run != 4 && t->fine_run != 8)
-        return fail(SVPF_ERR_ARG, "svpf_analyse_set_tuning: fine_run must be 1, 2, 4 or 8");
+    if (t->fine_run != 0 && t->fine_run != 1 && t->fine_run != 2 && t->fine_run != 4 && t->fine_run != 8 &&
+        t->fine_run != 16 && t->fine_run != 32)
+        return fail(SVPF_ERR_ARG, "svpf_analyse_set_tuning: fine_run must be 1, 2, 4, 8, 16 or 32");
     if (t->coarse_run != 0 && t->coarse_run != 4 && t->coarse_run != 8 && t->coarse_run != 16 && t->coarse_run != 32)
         return fail(SVPF_ERR_ARG, "svpf_analyse_set_tuning: coarse_run must be 4, 8, 16 or 32");
     if (t->chase > 65536 || t->slot_scale < 0.0 || t->slot_scale > 16.0 || t->coarse_slots < 0 ||

@@ -153,6 +153,7 @@ def test_analyse_sequential_fallback(gpu_ctx, oracle, max_sweeps):
 @pytest.mark.parametrize("tuning", [dict(chase=16), dict(chase=1), dict(chase=4096, sweep_grid=256),
                                     dict(fine_run=2), dict(fine_run=8, slot_scale=0.5), dict(fine_run=4, sweep_grid=512),
                                     dict(coarse_run=4, coarse_slots=64), dict(coarse_run=16, coarse_slots=4096),
+                                    dict(fine_run=32, slot_scale=2.8),
                                     dict(slot_scale=0.25, coarse_slots=512), dict(slot_scale=4.0, sweep_grid=64)])
 def test_analyse_tuning_keeps_vectors(gpu_ctx, oracle, tuning):
     """The search's schedule settings (chase along the visit order, slot counts, sweep
