@@ -167,7 +167,10 @@ int svpf_analyse_set_max_sweeps(svpf_ctx* ctx, int32_t max_sweeps);
  *   sweep_grid   workgroups of a later sweep (default 4096);
  *   fine_run     consecutive visit positions a finest-level sweep-0 item searches in
  *                order (1, 2, 4, 8, 16 or 32; default 1);
- *   coarse_run   the same at the coarser levels (4, 8, 16 or 32; default 16). */
+ *   coarse_run   the same at the coarser levels (4, 8, 16 or 32; default 16);
+ *   graphs       1 (default): the later sweeps of a level are replayed from a captured
+ *                hipGraph of six sweeps (cached per launch configuration); -1: launched
+ *                one by one.  The chase and svpf_analyse_pair_noalloc never use graphs. */
 typedef struct svpf_analyse_tuning {
     int32_t chase;
     double slot_scale;
@@ -175,6 +178,7 @@ typedef struct svpf_analyse_tuning {
     int32_t sweep_grid;
     int32_t fine_run;
     int32_t coarse_run;
+    int32_t graphs;
 } svpf_analyse_tuning;
 int svpf_analyse_set_tuning(svpf_ctx* ctx, const svpf_analyse_tuning* t);
 

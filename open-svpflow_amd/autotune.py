@@ -61,7 +61,8 @@ def apply(ctx: svpf.Context, setting: Dict[str, float]) -> None:
     s = dict(DEFAULT)
     s.update(setting)
     ctx.set_tuning(slot_scale=float(s["slot_scale"]), coarse_slots=int(s["coarse_slots"]),
-                   sweep_grid=int(s["sweep_grid"]), fine_run=int(s["fine_run"]), coarse_run=int(s["coarse_run"]))
+                   sweep_grid=int(s["sweep_grid"]), fine_run=int(s["fine_run"]), coarse_run=int(s["coarse_run"]),
+                   graphs=int(s.get("graphs", 0)))
 
 
 def descend(measure: Callable[[Dict[str, float]], float], space: Dict[str, Sequence] = SPACE,

@@ -1363,17 +1363,17 @@ __global__ void sweep_init_kernel(const Chain* __restrict__ chains, SweepState s
         st.minpos[blockIdx.y] = 0;  // the first rebuild covers the whole order
     }
     if (blockIdx.x == 0 && blockIdx.y == 0)
-        for (int k = threadIdx.x; k < st.max_sweeps + 2; k += blockDim.x) {
+        for (int k = threadIdx.x; k < kCntWords; k += blockDim.x) {
             st.counts[k] = 0;
             st.dcount[k] = 0;
         }
 }
 
-// Block n of chain ci into sweep s + 1's list wl[q][counts[s + 1]++], once (flags[q]).
+// Block n of chain ci into sweep s + 1's list wl[q][counts[(s + 1) % 3]++], once (flags[q]).
 __device__ __forceinline__ void queue_block(const SweepState& st, int ci, const Chain& ch, int n, int s) {
     const int q = (s + 1) & 1;
     if (atomicExch(&st.flags[q][ch.base + n], 1u) == 0u) {
-        const uint32_t pos = atomicAdd(&st.counts[s + 1], 1u);
+        const uint32_t pos = atomicAdd(&st.counts[(s + 1) % 3], 1u);
         st.wl[q][pos] = ((unsigned long long)ci << 32) | (unsigned)n;
     }
 }
@@ -1502,11 +1502,11 @@ __device__ __forceinline__ bool sweep_item(const Chain* __restrict__ chains, int
     const DVec o = search_block<false, kFast, kL0>(ch, g, b, idx, 0, &bad, &found, st.memo + ch.base + idx,
                                               st.seeds + ch.base + idx, &unchanged, RefineStart{}, &defer, &io);
     if (kFast && defer != kDeferNone) {
-        if (lane_id() == 0) st.dl[atomicAdd(&st.dcount[s], 1u)] = ((unsigned long long)ci << 32) | (unsigned)idx;
+        if (lane_id() == 0) st.dl[atomicAdd(&st.dcount[kDcDeferred], 1u)] = ((unsigned long long)ci << 32) | (unsigned)idx;
         if (defer == kDeferWhole) return false;
     }
     if (unchanged) {  // nothing it reads changed since its last evaluation
-        if (lane_id() == 0) atomicAdd(&st.counts[st.max_sweeps + 1], 1u);  // skip count (statistics)
+        if (lane_id() == 0) atomicAdd(&st.counts[kCntSkipped], 1u);  // skip count (statistics)
         return false;
     }
     // (the block's own words: no other evaluation stores them while this one runs)
@@ -1664,9 +1664,13 @@ __device__ __forceinline__ int order_successor(const Chain* __restrict__ chains,
 // sweep would cost more than three waves per SIMD gain.)
 template <bool kL0>
 __global__ void __launch_bounds__(kWave, 2) sweep_kernel(const Chain* __restrict__ chains, SweepState st, int s) {
-    const uint32_t n = (uint32_t)uniform_int((int)__atomic_load_n(&st.counts[s], __ATOMIC_RELAXED));
+    const uint32_t n = (uint32_t)uniform_int((int)__atomic_load_n(&st.counts[s % 3], __ATOMIC_RELAXED));
     const int p = s & 1;
     const int chase = st.chase;
+    if (blockIdx.x == 0 && lane_id() == 0) {
+        st.counts[(s + 2) % 3] = 0;  // read by sweep s - 1; sweep s + 1 queues into it
+        atomicAdd(&st.counts[kCntQueued], n);
+    }
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
         const unsigned long long item = st.wl[p][i];
         // uniform across the wave: keeps the chain's descriptors in scalar registers
@@ -1679,7 +1683,7 @@ __global__ void __launch_bounds__(kWave, 2) sweep_kernel(const Chain* __restrict
         const uint32_t base = (uint32_t)uniform_int((int)chains[ci].base);
         if (lane_id() == 0) st.flags[p][base + idx] = 0;
         if (!claim_block(st, base, idx, s)) {  // a chase of this sweep evaluated it
-            if (lane_id() == 0) atomicAdd(&st.counts[st.max_sweeps + 1], 1u);
+            if (lane_id() == 0) atomicAdd(&st.counts[kCntSkipped], 1u);
             continue;
         }
         for (int k = 0;; k++) {
@@ -1689,7 +1693,7 @@ __global__ void __launch_bounds__(kWave, 2) sweep_kernel(const Chain* __restrict
             if (next < 0 || !claim_block(st, base, next, s)) break;
             // this wave's store of idx before the successor reads it
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            if (lane_id() == 0) atomicAdd(&st.dcount[st.max_sweeps + 1], 1u);  // chased (statistics)
+            if (lane_id() == 0) atomicAdd(&st.dcount[kDcChased], 1u);  // chased (statistics)
             idx = next;
         }
     }
@@ -1703,7 +1707,7 @@ __global__ void __launch_bounds__(kWave, 2) sweep_kernel(const Chain* __restrict
 // evaluated again -- holds as for the others.
 __global__ void __launch_bounds__(kWave, 2) deferred_kernel(const Chain* __restrict__ chains, SweepState st,
                                                             int s) {
-    const uint32_t n = (uint32_t)uniform_int((int)__atomic_load_n(&st.dcount[s], __ATOMIC_RELAXED));
+    const uint32_t n = (uint32_t)uniform_int((int)__atomic_load_n(&st.dcount[kDcDeferred], __ATOMIC_RELAXED));
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
         const unsigned long long item = st.dl[i];
         const int ci = __builtin_amdgcn_readfirstlane((int)(item >> 32));
@@ -1714,28 +1718,20 @@ __global__ void __launch_bounds__(kWave, 2) deferred_kernel(const Chain* __restr
 
 // Statistics of one converged level run (profiling): out4 += {1 run, `sweeps` launched,
 // list items not evaluated -- nothing the block reads had changed, or a chase of the
-// same sweep had evaluated it (counts[max_sweeps + 1]) --, evaluations run (every block
-// in sweep 0 plus the lists of sweeps 1 .. sweeps - 1 and the chased blocks
-// (dcount[max_sweeps + 1]), less those not evaluated)}.  Launches on one stream are
-// ordered, so plain adds suffice.
-__global__ void __launch_bounds__(256) sweep_stats_kernel(SweepState st, int sweeps, long long total_blocks,
-                                                          unsigned long long* __restrict__ out4) {
-    __shared__ unsigned long long part[256];
-    unsigned long long v = 0;
-    for (int k = 1 + (int)threadIdx.x; k < sweeps; k += 256) v += st.counts[k];
-    part[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
-        __syncthreads();
-    }
+// same sweep had evaluated it (counts[kCntSkipped]) --, evaluations run (every block in
+// sweep 0 plus the items of the later sweeps (counts[kCntQueued]) and the chased blocks
+// (dcount[kDcChased]), less those not evaluated)}.  Launches on one stream are ordered,
+// so plain adds suffice.
+__global__ void __launch_bounds__(64) sweep_stats_kernel(SweepState st, int sweeps, long long total_blocks,
+                                                         unsigned long long* __restrict__ out4) {
     if (threadIdx.x == 0) {
-        const unsigned long long skipped = st.counts[st.max_sweeps + 1];
-        const unsigned long long chased = st.dcount[st.max_sweeps + 1];
+        const unsigned long long queued = st.counts[kCntQueued];
+        const unsigned long long skipped = st.counts[kCntSkipped];
+        const unsigned long long chased = st.dcount[kDcChased];
         out4[0] += 1;
         out4[1] += (unsigned long long)sweeps;
         out4[2] += skipped;
-        out4[3] += (unsigned long long)total_blocks + part[0] + chased - skipped;
+        out4[3] += (unsigned long long)total_blocks + queued + chased - skipped;
     }
 }
 
@@ -2202,7 +2198,7 @@ hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, i
 
 hipError_t launch_sweep_stats(const SweepState& st, int sweeps, long long total_blocks, unsigned long long* out4,
                               hipStream_t s) {
-    hipLaunchKernelGGL(sweep_stats_kernel, dim3(1), dim3(256), 0, s, st, sweeps, total_blocks, out4);
+    hipLaunchKernelGGL(sweep_stats_kernel, dim3(1), dim3(64), 0, s, st, sweeps, total_blocks, out4);
     return hipGetLastError();
 }
 

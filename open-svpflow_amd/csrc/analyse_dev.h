@@ -89,10 +89,18 @@ struct alignas(16) SeedRec {
 // visited before it (`cpre`, rebuilt after every sweep from the first visit position
 // whose found/bad changed, `minpos` per chain, with `fseq` as the rebuild's copy of
 // the costs in visit order).
+constexpr int kCntQueued = 3, kCntSkipped = 4, kCntWords = 8;
+constexpr int kDcDeferred = 0, kDcChased = 1;
+
 struct SweepState {
     unsigned long long* wl[2];  // items: chain index << 32 | block raster index
     uint32_t* flags[2];         // per block (chain base + raster index): queued for that parity
-    uint32_t* counts;           // counts[s] = items of sweep s (counts[0] unused)
+    // counts[s % 3] = items queued for sweep s >= 1 (a ring: sweep s reads its slot,
+    // queues into the next and clears the one after, which sweep s - 1 read), so every
+    // sweep with the same s mod 6 launches with the same arguments and parity;
+    // counts[kCntQueued] = items of all later sweeps, counts[kCntSkipped] = items not
+    // evaluated (statistics)
+    uint32_t* counts;
     uint32_t* found;            // per block
     uint32_t* badf;             // per block
     int32_t* cpre;              // per block
@@ -100,7 +108,7 @@ struct SweepState {
     Memo* memo;                 // per block
     SeedRec* seeds;             // per block
     unsigned long long* dl;     // blocks the tile-plan kernels passed on in the current sweep
-    uint32_t* dcount;           // dcount[s] = their number in sweep s
+    uint32_t* dcount;           // dcount[kDcDeferred]: their number; dcount[kDcChased]: chased evaluations
     uint32_t* fseq;             // per block: pre-expansion costs in visit order (count rebuild)
     uint32_t* stamp;            // per block: the last sweep (>= 1) whose evaluation claimed it
     int max_sweeps;

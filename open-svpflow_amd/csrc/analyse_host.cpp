@@ -155,16 +155,77 @@ constexpr int kMaxSweeps = 4096;
 constexpr int kNotConverged = 1;
 
 
+constexpr int kGraphSweeps = 6;  // sweeps per captured graph: lists by s mod 2, queue lengths by s mod 3
+constexpr size_t kSweepGraphs = 16;
+
+// The executable graph of kGraphSweeps later sweeps (each followed by its count rebuild
+// when `scan`) for these launch arguments, captured on first use.  Sweep s reads its
+// list by s mod 2 and its queue length by s mod 3, so the graph captured for sweeps
+// 1 .. 6 replays sweeps 6k + 1 .. 6k + 6 exactly.
+int sweep_graph(svpf_ctx* ctx, const Chain* dch, int nch, int grid, const SweepState& st, bool scan, bool l0,
+                hipGraphExec_t* out) {
+    struct Key {
+        const Chain* dch;
+        SweepState st;
+        int nch, grid, scan, l0;
+    } k;
+    std::memset(&k, 0, sizeof k);  // padding bytes take part in the comparison
+    k.dch = dch;
+    k.st = st;
+    k.nch = nch;
+    k.grid = grid;
+    k.scan = scan;
+    k.l0 = l0;
+    const std::string key(reinterpret_cast<const char*>(&k), sizeof k);
+    for (auto& g : ctx->graphs)
+        if (g.key == key) {
+            g.used = ++ctx->graph_clock;
+            *out = g.exec;
+            return SVPF_OK;
+        }
+    if (ctx->graphs.size() >= kSweepGraphs) {  // evict the least recently used
+        size_t v = 0;
+        for (size_t i = 1; i < ctx->graphs.size(); i++)
+            if (ctx->graphs[i].used < ctx->graphs[v].used) v = i;
+        SVPF_HIP(hipStreamSynchronize(ctx->stream));  // it may still be running
+        (void)hipGraphExecDestroy(ctx->graphs[v].exec);
+        (void)hipGraphDestroy(ctx->graphs[v].graph);
+        ctx->graphs.erase(ctx->graphs.begin() + (long)v);
+    }
+    svpf_ctx::SweepGraph g;
+    g.key = key;
+    SVPF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int j = 1; j <= kGraphSweeps && e == hipSuccess; j++) {
+        e = launch_sweep(dch, grid, st, j, ctx->stream, l0);
+        if (e == hipSuccess && scan) e = launch_sweep_scan(dch, nch, st, j, ctx->stream);
+    }
+    const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g.graph);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+        return fail(SVPF_ERR_HIP, std::string("svpf_analyse: sweep graph capture: ") + hipGetErrorString(e));
+    }
+    g.used = ++ctx->graph_clock;
+    ctx->graphs.push_back(g);
+    *out = g.exec;
+    return SVPF_OK;
+}
+
 // Parallel sweeps of one level until no block changes (each sweep evaluates the blocks
 // whose earlier-visited neighbours changed in the previous sweep).  The fixed point is
 // unique -- every block depends only on blocks visited before it -- and equals the
-// sequential result.  The host checks the queue length after each chunk of sweeps;
-// sweeps launched past convergence find an empty queue and return at once.  fast: every
-// chain's searched block is a 4x4 or 8x8 square, and sweep 0 (every block of the level)
-// runs the tile-plan kernel, then the general one over the blocks it passed on.
+// sequential result.  The later sweeps go in chunks: replays of a captured graph of
+// kGraphSweeps sweeps (or, with the chase, without graphs or in the allocation-free
+// entry point, launches one by one); the host reads the next queue length after each
+// chunk, and sweeps launched past convergence find an empty queue and return at once.
+// fast: every chain's searched block is a 4x4 or 8x8 square, and sweep 0 (every block
+// of the level) runs the tile-plan kernel, then the general one over the blocks it
+// passed on.
 int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total_blocks, const SweepState& st,
                bool scan, int level, bool fast, bool l0, bool no_alloc) {
-    const size_t words = (size_t)kMaxSweeps + 2;
+    const size_t words = 4;
     uint32_t local_count = 0;  // no_alloc: read back through pageable memory (a synchronous copy)
     uint32_t* h_count = no_alloc ? &local_count : nullptr;
     if (!no_alloc && ctx->h_count_cap < words) {
@@ -181,15 +242,25 @@ int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total
     if (fast) SVPF_HIP(launch_deferred(dch, grid, st, 0, ctx->stream));
     if (scan) SVPF_HIP(launch_sweep_scan(dch, nch, st, 0, ctx->stream));
     SVPF_HIP(launch_validate(dch, nch, max_n, st, ctx->stream));
-    int s = 1, chunk = 8;
+    // the chase claims blocks by absolute sweep number, so its sweeps are not replayable
+    hipGraphExec_t exec = nullptr;
+    if (ctx->sweep_graphs && !no_alloc && st.chase == 0) {
+        const int rc = sweep_graph(ctx, dch, nch, grid, st, scan, l0, &exec);
+        if (rc) return rc;
+    }
+    int s = 1, chunk = exec ? kGraphSweeps : 8;
     for (;;) {
         if (s + chunk > st.max_sweeps) return kNotConverged;
-        for (int k = 0; k < chunk; k++) {
-            SVPF_HIP(launch_sweep(dch, grid, st, s + k, ctx->stream, l0));
-            if (scan) SVPF_HIP(launch_sweep_scan(dch, nch, st, s + k, ctx->stream));
+        if (exec) {
+            for (int k = 0; k < chunk; k += kGraphSweeps) SVPF_HIP(hipGraphLaunch(exec, ctx->stream));
+        } else {
+            for (int k = 0; k < chunk; k++) {
+                SVPF_HIP(launch_sweep(dch, grid, st, s + k, ctx->stream, l0));
+                if (scan) SVPF_HIP(launch_sweep_scan(dch, nch, st, s + k, ctx->stream));
+            }
         }
         s += chunk;
-        SVPF_HIP(hipMemcpyAsync(h_count, st.counts + s, 4, hipMemcpyDeviceToHost, ctx->stream));
+        SVPF_HIP(hipMemcpyAsync(h_count, st.counts + s % 3, 4, hipMemcpyDeviceToHost, ctx->stream));
         SVPF_HIP(hipStreamSynchronize(ctx->stream));
         if (*h_count == 0) {
             if (ctx->prof)  // statistics for the benchmark, added on the device
@@ -197,7 +268,7 @@ int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total
                                             ctx->d_sweep_stats + 4 * std::min(level, 15), ctx->stream));
             return SVPF_OK;
         }
-        chunk = std::min(chunk * 2, 64);
+        chunk = std::min(chunk * 2, exec ? 10 * kGraphSweeps : 64);
     }
 }
 
@@ -350,7 +421,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
             flags_off[l][q] = work.take(4 * tot + 4);
             wl_off[l][q] = work.take(8 * tot + 8);
         }
-        counts_off[l] = work.take(8 * (size_t)(kMaxSweeps + 2));  // counts, then dcount
+        counts_off[l] = work.take(8 * (size_t)kCntWords);  // counts, then dcount
         dl_off[l] = work.take(8 * tot + 8);
         found_off[l] = work.take(4 * tot + 4);
         badf_off[l] = work.take(4 * tot + 4);
@@ -703,7 +774,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
                 st.flags[q] = (uint32_t*)(W + flags_off[l][q]);
             }
             st.counts = (uint32_t*)(W + counts_off[l]);
-            st.dcount = st.counts + kMaxSweeps + 2;
+            st.dcount = st.counts + kCntWords;
             st.dl = (unsigned long long*)(W + dl_off[l]);
             st.found = (uint32_t*)(W + found_off[l]);
             st.badf = (uint32_t*)(W + badf_off[l]);
@@ -890,7 +961,8 @@ int svpf_analyse_set_tuning(svpf_ctx* ctx, const svpf_analyse_tuning* t) {
     if (t->coarse_run != 0 && t->coarse_run != 4 && t->coarse_run != 8 && t->coarse_run != 16 && t->coarse_run != 32)
         return fail(SVPF_ERR_ARG, "svpf_analyse_set_tuning: coarse_run must be 4, 8, 16 or 32");
     if (t->chase > 65536 || t->slot_scale < 0.0 || t->slot_scale > 16.0 || t->coarse_slots < 0 ||
-        t->coarse_slots > (1 << 20) || t->sweep_grid < 0 || t->sweep_grid > (1 << 20))
+        t->coarse_slots > (1 << 20) || t->sweep_grid < 0 || t->sweep_grid > (1 << 20) || t->graphs < -1 ||
+        t->graphs > 1)
         return fail(SVPF_ERR_ARG, "svpf_analyse_set_tuning: setting out of range");
     ctx->chase = std::max(t->chase, 0);
     ctx->slot_scale = t->slot_scale == 0.0 ? 1.0 : t->slot_scale;
@@ -898,6 +970,7 @@ int svpf_analyse_set_tuning(svpf_ctx* ctx, const svpf_analyse_tuning* t) {
     ctx->coarse_slots = t->coarse_slots == 0 ? 256 : t->coarse_slots;
     ctx->coarse_run = t->coarse_run == 0 ? 16 : t->coarse_run;
     ctx->sweep_grid = t->sweep_grid == 0 ? 4096 : t->sweep_grid;
+    ctx->sweep_graphs = t->graphs >= 0;
     return SVPF_OK;
 }
 

@@ -175,6 +175,10 @@ void svpf_destroy(svpf_ctx* ctx) {
     prof_clear(ctx);
     for (auto e : ctx->prof_pool) (void)hipEventDestroy(e);
     if (ctx->h_count) (void)hipHostFree(ctx->h_count);
+    for (auto& g : ctx->graphs) {
+        (void)hipGraphExecDestroy(g.exec);
+        (void)hipGraphDestroy(g.graph);
+    }
     if (ctx->d_sweep_stats) (void)hipFree(ctx->d_sweep_stats);
     (void)hipEventDestroy(ctx->desc_done);
     (void)hipEventDestroy(ctx->t0);

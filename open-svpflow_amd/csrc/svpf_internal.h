@@ -36,6 +36,17 @@ struct svpf_ctx {
     int coarse_slots = 256;    // sweep-0 slots of the coarser levels
     int coarse_run = 16;       // coarser-level positions per sweep-0 item
     int sweep_grid = 4096;     // workgroups of a later sweep
+    // Analyse: later sweeps replayed from captured hipGraphs (one per level launch
+    // configuration, least recently used evicted beyond kSweepGraphs).
+    bool sweep_graphs = true;
+    struct SweepGraph {
+        std::string key;  // the launch arguments the graph was captured with
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        unsigned long long used = 0;
+    };
+    std::vector<SweepGraph> graphs;
+    unsigned long long graph_clock = 0;
     // Analyse sweep statistics per level since profiling was last enabled (device, 16 x 4,
     // added by a kernel so that profiling adds no host round trip):
     // {level runs, sweeps launched, evaluations skipped, evaluations run}

@@ -154,7 +154,8 @@ def test_analyse_sequential_fallback(gpu_ctx, oracle, max_sweeps):
                                     dict(fine_run=2), dict(fine_run=8, slot_scale=0.5), dict(fine_run=4, sweep_grid=512),
                                     dict(coarse_run=4, coarse_slots=64), dict(coarse_run=16, coarse_slots=4096),
                                     dict(fine_run=32, slot_scale=2.8),
-                                    dict(slot_scale=0.25, coarse_slots=512), dict(slot_scale=4.0, sweep_grid=64)])
+                                    dict(slot_scale=0.25, coarse_slots=512), dict(slot_scale=4.0, sweep_grid=64),
+                                    dict(graphs=-1), dict(graphs=-1, sweep_grid=128)])
 def test_analyse_tuning_keeps_vectors(gpu_ctx, oracle, tuning):
     """The search's schedule settings (chase along the visit order, slot counts, sweep
     grid) change only how the level's fixed point is reached, not the vectors."""
@@ -165,3 +166,40 @@ def test_analyse_tuning_keeps_vectors(gpu_ctx, oracle, tuning):
         _check(gpu_ctx, oracle, frames, "{pel:2}", CFG2, batch=True)
     finally:
         gpu_ctx.set_tuning()
+
+
+def test_analyse_graph_replays_match_launches(gpu_ctx, oracle):
+    """The later sweeps replayed from a captured hipGraph (cached per launch
+    configuration, so later calls replay the graph captured by the first) give the
+    payloads of the same sweeps launched one by one, call after call, with other work
+    and synchronisations on the stream between the replays."""
+    print("analyse parity: sweep graph replays vs launches", flush=True)
+    frames = synth.clip(416, 240, 4, seed=4242, vmax=10.0, nobj=(2, 4), rot_deg=1.5)
+    h, w = frames[0][0].shape
+    sdata, devs, hosts = _supers(gpu_ctx, oracle, frames, "{pel:2}")
+    p, vdata, plen = svpf.analyse_params(CFG2, sdata, w, h)
+    pays = [gpu_ctx.alloc(plen) for _ in range(len(frames) - 1)]
+    gpu_ctx.profile(True)
+    try:
+        gpu_ctx.set_tuning(graphs=-1)
+        svpf.analyse_batch(gpu_ctx, p, devs[:-1], devs[1:], pays)
+        gpu_ctx.sync()
+        eager = [gpu_ctx.download(q, plen) for q in pays]
+        for i in range(len(frames) - 1):
+            ref = oracle.analyse(CFG2, sdata, w, h, hosts[i], hosts[i + 1])
+            assert np.array_equal(eager[i], ref), f"pair {i}: launches differ from the oracle"
+        gpu_ctx.set_tuning()
+        for call in range(3):
+            for q in pays:  # other work on the stream between the calls
+                gpu_ctx.memset(q, 0xA5, plen)
+            svpf.analyse_pair(gpu_ctx, p, devs[0], devs[1], pays[0])  # its own graphs
+            gpu_ctx.sync()
+            svpf.analyse_batch(gpu_ctx, p, devs[:-1], devs[1:], pays)
+            gpu_ctx.sync()
+            for i, q in enumerate(pays):
+                assert np.array_equal(gpu_ctx.download(q, plen), eager[i]), f"call {call}, pair {i}: graph replay differs"
+        st = gpu_ctx.analyse_stats(0)
+        assert st["runs"] >= 4 and st["sweeps"] > st["runs"], st
+    finally:
+        gpu_ctx.set_tuning()
+        gpu_ctx.profile(False)
