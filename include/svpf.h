@@ -219,7 +219,9 @@ int svpf_smooth_set_sources(svpf_smooth* s, int32_t first, int32_t count, const 
                             const svpf_frame* d_super, uint8_t* const* d_vectors);
 /* Renders output frames k0..k0+count-1 into d_out[0..count-1] in one batch of
  * launches on the context's stream (get_frame -> try_render -> render_into_output,
- * core.rs:752-1770).  If h_decision is not NULL it receives, per frame, the pair
+ * core.rs:752-1770), each of svpf_smooth_output_size (with light.* padding, the light
+ * border of output frame k is drawn around it: light.rs, core.rs:1866-1897).  If
+ * h_decision is not NULL it receives, per frame, the pair
  * (kind, algo): kind 0 = copy of source n, 1 = copy of n+1, 2 = interpolated with
  * `algo`; the call then synchronizes. */
 int svpf_smooth_render(svpf_smooth* s, int32_t k0, int32_t count, const svpf_frame* d_out, int32_t* h_decision);
@@ -234,6 +236,11 @@ int svpf_smooth_render(svpf_smooth* s, int32_t k0, int32_t count, const svpf_fra
  * scene blending (the reference falls back only for their zero-origin frames). */
 int svpf_smooth_set_float_render(svpf_smooth* s, int32_t on);
 int32_t svpf_smooth_output_frames(const svpf_smooth* s);
+/* output_info (crates/svpflow2/src/core.rs:603-613): the output frame size, the source
+ * size plus twice the light.* padding (Options::padding, options.rs:256-277).  Output
+ * frames passed to svpf_smooth_render have this size; the rendered frame sits at
+ * (pad_x, pad_y) (chroma at half) and the light border (light.rs) fills the rest. */
+int svpf_smooth_output_size(const svpf_smooth* s, int32_t* width, int32_t* height);
 /* Output frame rate of the session (FilterState::output_info, core.rs:603-613). */
 int svpf_smooth_output_rate(const svpf_smooth* s, int64_t* fps_num, int64_t* fps_den);
 /* What output frame k reads, for a host that fetches frames on demand (the

@@ -32,6 +32,16 @@ struct SmoothGeom {
     int32_t lim_zero, lim_m1, lim_m2, lim_scene, lim_blocks;
     int32_t nframes;
     int32_t float_render;            // rendered frames through the float renderer (svpf_smooth_set_float_render)
+    // light border (Options::padding and LightParams, options.rs:256-286; light.rs):
+    // pads per side of the luma plane (chroma: half), zero without a border
+    int32_t pad_x, pad_y;
+    int32_t light_border, light_lights, light_length;
+    double light_cell;
+    int32_t light_slots;             // border profiles per frame over the three planes
+    int32_t light_prof_len;          // floats of one frame's profiles
+    int32_t light_draws;             // random draws of an even frame (the most a frame takes)
+    int32_t light_pixels;            // border pixels per frame over the three planes
+    const double* corner_geo[2];     // (mix, weight) per corner cell: luma pad_x x pad_y, chroma half
     float lin_lut[256];              // linear-light luma of each 8-bit sample (the float renderer's luma image)
     uint8_t luma_lut[512];           // luma_byte((a + b) / denom, gamma) for a + b in 0..510
     uint8_t cover_lut[4097];         // coverage_byte for remaining = 0..area (area <= 4096)
@@ -80,8 +90,15 @@ struct FrameJob {
     const uint8_t* sup0[3];    // super planes of n / n+1
     const uint8_t* sup1[3];
     int32_t supstride[3];
-    uint8_t* dst[3];
+    uint8_t* dst[3];           // the output frame's interior (the whole frame without padding)
     int32_t dstride[3];
+    uint8_t* obase[3];         // the padded output frame (light border)
+    float* prof;               // its border profiles
+    int32_t* draws;            // its random draws
 };
+
+// Limits of the light border: profile length (frame side) and control points.
+constexpr int kLightMaxDim = 8192;
+constexpr int kLightMaxLights = 4096;
 
 }  // namespace svpf

@@ -21,6 +21,8 @@ hipError_t launch_smooth_render(const FrameJob* d_jobs, const FramePlan* d_plans
                                 int width, int height, int mw, hipStream_t s);
 hipError_t launch_smooth_render_float(const FrameJob* d_jobs, const FramePlan* d_plans, int njobs,
                                       const SmoothGeom* d_geom, int width, int height, hipStream_t s);
+hipError_t launch_smooth_light(const FrameJob* d_jobs, const FramePlan* d_plans, int njobs, const SmoothGeom* d_geom,
+                               const SmoothGeom& g, hipStream_t s);
 
 namespace {
 
@@ -193,6 +195,61 @@ void light_padding(const Opt* op, int width, int height, int* pad_x, int* pad_y)
     *pad_y = (hy + (hy + 1 < 0 ? 1 : 0) + 1) & ~1;
 }
 
+// LightParams (options.rs:279-286): border 12, lights 16 (at least 1), length 100, cell 1.0
+// (at least 0.1).
+struct LightSet {
+    int border, lights, length;
+    double cell;
+};
+LightSet light_params(const Opt* op) {
+    int64_t v;
+    double f;
+    auto gi = [&](std::initializer_list<const char*> k, int64_t d) { return (op && op->get_int(k, &v)) ? v : d; };
+    auto sat = [](int64_t x) { return (int)std::min<int64_t>(std::max<int64_t>(x, INT32_MIN), INT32_MAX); };
+    LightSet l;
+    l.border = sat(gi({"light", "border"}, 12));
+    l.lights = std::max(sat(gi({"light", "lights"}, 16)), 1);
+    l.length = sat(gi({"light", "length"}, 100));
+    l.cell = std::max((op && op->get_float({"light", "cell"}, &f)) ? f : 1.0, 0.1);
+    return l;
+}
+
+// Sizes this build draws the light border for (each a deviation only for settings far
+// outside the reference's use): frames of 4 to 8192 pixels a side, border <= 255 samples
+// and no deeper than the frame, at most 4096 control points a side, cell <= 1000.
+const char* light_limits(const Opt* op, int width, int height, int pad_x, int pad_y) {
+    const LightSet l = light_params(op);
+    if (pad_x < 0 || pad_y < 0) return "SVSmoothFps: light padding would shrink the frame";
+    if (width < 4 || height < 4 || width > kLightMaxDim || height > kLightMaxDim)
+        return "SVSmoothFps: light border needs a frame of 4 to 8192 pixels a side";
+    if (l.border > 255) return "SVSmoothFps: light.border above 255 is not supported";
+    if (!(l.cell <= 1000.0)) return "SVSmoothFps: light.cell above 1000 is not supported";
+    for (int p = 0; p < 2; p++) {
+        const int w = p ? width / 2 : width, h = p ? height / 2 : height;
+        const int b = std::max(p ? l.border / 2 : l.border, 1);
+        if ((pad_y > 0 && b > h) || (pad_x > 0 && b > w)) return "SVSmoothFps: light.border deeper than the frame";
+        if (l.lights > kLightMaxLights || (pad_x > 0 && (int64_t)l.lights * h / w > kLightMaxLights))
+            return "SVSmoothFps: light.lights above 4096 a side is not supported";
+    }
+    return nullptr;
+}
+
+// corner_geometry (light.rs:350-373) of a w x h corner as (mix, weight) pairs, built here
+// with the host's IEEE double arithmetic and sqrt, as the reference builds it.
+std::vector<double> corner_table(int w, int h, int length) {
+    std::vector<double> t;
+    t.reserve(2 * (size_t)w * h);
+    for (int i = 0; i < h; i++)
+        for (int j = 0; j < w; j++) {
+            const double x = (double)(j + 1) / (double)w;
+            const double y = (double)(i + 1) / (double)h;
+            const double distance = std::sqrt(x * x + y * y) * 100.0;
+            t.push_back(x > y ? y / x * 0.5 : (y >= 0.0001 ? 1.0 - x / y * 0.5 : 1.0));
+            t.push_back(length > 0 ? std::max(1.0 - distance / (double)length, 0.0) : 0.0);
+        }
+    return t;
+}
+
 // scale_magnitude (renderer.rs:3245-3259) for every 24-bit magnitude, evaluated with this
 // host's libm pow in the reference's operation order.  The reference computes the mask on
 // the CPU with the same expression, so the table is that computation, not an
@@ -228,6 +285,7 @@ struct svpf_smooth {
     SmoothGeom geom;
     SmoothGeom* d_geom = nullptr;
     uint8_t* d_mag_lut = nullptr;
+    double* d_corner = nullptr;  // light border corner tables (luma, chroma)
     int count, mw, mh, mc;
     std::vector<svpf_frame> src, sup;
     std::vector<uint8_t*> vec;
@@ -257,7 +315,7 @@ PrepLayout prep_layout(const svpf_smooth* s) {
     return L;
 }
 struct FrameLayout {
-    size_t acc[2], cov[2], maxmag, total;
+    size_t acc[2], cov[2], maxmag, prof, draws, total;
 };
 FrameLayout frame_layout(const svpf_smooth* s) {
     FrameLayout L{};
@@ -266,6 +324,8 @@ FrameLayout frame_layout(const svpf_smooth* s) {
     for (int k = 0; k < 2; k++) L.acc[k] = take(4 * (size_t)(s->mw + 2) * (s->mh + 2));
     for (int k = 0; k < 2; k++) L.cov[k] = take((size_t)s->mc);
     L.maxmag = take((size_t)s->mc);
+    L.prof = take(4 * (size_t)s->geom.light_prof_len);
+    L.draws = take(4 * (size_t)s->geom.light_draws);
     L.total = off;
     return L;
 }
@@ -354,10 +414,13 @@ int svpf_smooth_create(svpf_ctx* ctx, const char* opt, size_t opt_len, int32_t w
     int pad_x, pad_y;
     light_padding(have ? &o : nullptr, width, height, &pad_x, &pad_y);
     if (pad_x != 0 || pad_y != 0) {
-        // The reference widens the output by 2*padding and draws its light border there
-        // (core.rs:603-613, 1866-1897, light.rs); this build renders only unpadded output.
-        delete s;
-        return reject("SVSmoothFps: light.aspect/light.zoom padding is not supported by this build");
+        // the output is widened by the padding and the light border drawn there (core.rs
+        // :603-613, 1866-1897, light.rs); the limits keep every profile inside the frame
+        // and its sums in 32 bits
+        if (const char* e = light_limits(have ? &o : nullptr, width, height, pad_x, pad_y)) {
+            delete s;
+            return reject(e);
+        }
     }
     s->timing = s->opt.timing(fps_num, fps_den);
     s->width = width;
@@ -452,11 +515,45 @@ int svpf_smooth_create(svpf_ctx* ctx, const char* opt, size_t opt_len, int32_t w
         }
         g.cover_lut[r] = (uint8_t)b;
     }
+    // light border geometry (light.rs:54-235): per plane two profiles per padded side,
+    // one random draw per strip row and per corner row after 15 skipped (even frames) and
+    // the six noise tables
+    g.pad_x = pad_x;
+    g.pad_y = pad_y;
+    std::vector<double> corners[2];
+    if (pad_x || pad_y) {
+        const LightSet l = light_params(have ? &o : nullptr);
+        g.light_border = l.border;
+        g.light_lights = l.lights;
+        g.light_length = l.length;
+        g.light_cell = l.cell;
+        g.light_draws = 15 + 6 * 128;
+        for (int p = 0; p < 3; p++) {
+            const int w = p ? width / 2 : width, h = p ? height / 2 : height;
+            const int px = p ? pad_x / 2 : pad_x, py = p ? pad_y / 2 : pad_y;
+            if (py > 0) { g.light_slots += 2; g.light_prof_len += 2 * w; g.light_draws += 2 * py; }
+            if (px > 0) { g.light_slots += 2; g.light_prof_len += 2 * h; g.light_draws += 2 * px; }
+            if (px > 0 && py > 0) g.light_draws += 4 * py;
+            g.light_pixels += (w + 2 * px) * (h + 2 * py) - w * h;
+        }
+        if (pad_x > 0 && pad_y > 0) {
+            corners[0] = corner_table(pad_x, pad_y, l.length);
+            corners[1] = corner_table(pad_x / 2, pad_y / 2, l.length);
+        }
+    }
     s->count = g.count;
     s->mw = g.mw;
     s->mh = g.mh;
     s->mc = g.mw * g.mh;
     hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess && !corners[0].empty()) {
+        const size_t n0 = corners[0].size() * sizeof(double), n1 = corners[1].size() * sizeof(double);
+        e = hipMalloc(&s->d_corner, n0 + n1);
+        if (e == hipSuccess) e = hipMemcpy(s->d_corner, corners[0].data(), n0, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(s->d_corner + corners[0].size(), corners[1].data(), n1, hipMemcpyHostToDevice);
+        g.corner_geo[0] = s->d_corner;
+        g.corner_geo[1] = s->d_corner + corners[0].size();
+    }
     if (e == hipSuccess && g.mask_area && g.area_sharp != 1.0) {
         const std::vector<uint8_t> t = magnitude_table(area, g.area_scale, g.area_sharp);
         e = hipMalloc(&s->d_mag_lut, t.size());
@@ -467,6 +564,7 @@ int svpf_smooth_create(svpf_ctx* ctx, const char* opt, size_t opt_len, int32_t w
     if (e == hipSuccess) e = hipMemcpy(s->d_geom, &g, sizeof g, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         if (s->d_mag_lut) (void)hipFree(s->d_mag_lut);
+        if (s->d_corner) (void)hipFree(s->d_corner);
         if (s->d_geom) (void)hipFree(s->d_geom);
         delete s;
         return fail(SVPF_ERR_HIP, std::string("svpf_smooth_create: ") + hipGetErrorString(e));
@@ -489,6 +587,7 @@ void svpf_smooth_destroy(svpf_smooth* s) {
     if (s->scratch) (void)hipFree(s->scratch);
     if (s->d_geom) (void)hipFree(s->d_geom);
     if (s->d_mag_lut) (void)hipFree(s->d_mag_lut);
+    if (s->d_corner) (void)hipFree(s->d_corner);
     delete s;
 }
 
@@ -632,8 +731,19 @@ int svpf_smooth_render(svpf_smooth* s, int32_t k0, int32_t count, const svpf_fra
         }
         const svpf_frame& d = d_out[i];
         if (!d.y || !d.u || !d.v) return fail(SVPF_ERR_ARG, "svpf_smooth_render: output frame not set");
-        j.dst[0] = d.y; j.dst[1] = d.u; j.dst[2] = d.v;
+        const int pad_x = g.pad_x, pad_y = g.pad_y;
+        if (d.stride_y < s->width + 2 * pad_x || d.stride_u < s->width / 2 + pad_x ||
+            d.stride_v < s->width / 2 + pad_x)
+            return fail(SVPF_ERR_ARG, "svpf_smooth_render: output stride below the output width");
+        // the frame goes at (pad_x, pad_y) of a padded output (chroma at half), the light
+        // border around it
+        j.obase[0] = d.y; j.obase[1] = d.u; j.obase[2] = d.v;
         j.dstride[0] = d.stride_y; j.dstride[1] = d.stride_u; j.dstride[2] = d.stride_v;
+        j.dst[0] = d.y + (size_t)pad_y * d.stride_y + pad_x;
+        j.dst[1] = d.u + (size_t)(pad_y / 2) * d.stride_u + pad_x / 2;
+        j.dst[2] = d.v + (size_t)(pad_y / 2) * d.stride_v + pad_x / 2;
+        j.prof = (float*)(fs + FL.prof);
+        j.draws = (int32_t*)(fs + FL.draws);
     }
     void* d_up = nullptr;
     rc = svpf_upload_desc(s->ctx, blob.data(), blob.size(), &d_up);
@@ -650,6 +760,7 @@ int svpf_smooth_render(svpf_smooth* s, int32_t k0, int32_t count, const svpf_fra
     else
         SVPF_HIP(launch_smooth_render((const FrameJob*)d_up, d_plans, count, s->d_geom, s->width, s->height, s->mw,
                                       s->ctx->stream));
+    SVPF_HIP(launch_smooth_light((const FrameJob*)d_up, d_plans, count, s->d_geom, g, s->ctx->stream));
     prof_end(s->ctx, ph);
     if (h_decision) {  // (decision, algo) per frame; synchronizes
         SVPF_HIP(hipMemcpyAsync(plans.data(), d_plans, (size_t)count * sizeof(FramePlan), hipMemcpyDeviceToHost,
@@ -676,6 +787,13 @@ int svpf_smooth_set_float_render(svpf_smooth* s, int32_t on) {
 }
 
 int32_t svpf_smooth_output_frames(const svpf_smooth* s) { return s ? s->out_frames : 0; }
+
+int svpf_smooth_output_size(const svpf_smooth* s, int32_t* width, int32_t* height) {
+    if (!s || !width || !height) return fail(SVPF_ERR_ARG, "svpf_smooth_output_size: bad argument");
+    *width = s->width + 2 * s->geom.pad_x;  // output_info (core.rs:603-613)
+    *height = s->height + 2 * s->geom.pad_y;
+    return SVPF_OK;
+}
 
 int svpf_smooth_output_rate(const svpf_smooth* s, int64_t* fps_num, int64_t* fps_den) {
     if (!s || !fps_num || !fps_den) return fail(SVPF_ERR_ARG, "svpf_smooth_output_rate: bad argument");

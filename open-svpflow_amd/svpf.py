@@ -334,6 +334,7 @@ def _bind_smooth(L):
     L.svpf_smooth_render.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(Frame), C.POINTER(C.c_int32)]
     L.svpf_smooth_output_frames.argtypes = [C.c_void_p]
     L.svpf_smooth_output_frames.restype = C.c_int32
+    L.svpf_smooth_output_size.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.svpf_smooth_release.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
     L.svpf_smooth_set_float_render.argtypes = [C.c_void_p, C.c_int32]
 
@@ -364,6 +365,9 @@ class Smooth:
         check(rc)
         self.h = h
         self.output_frames = nout.value
+        ow, oh = C.c_int32(), C.c_int32()
+        check(L.svpf_smooth_output_size(h, C.byref(ow), C.byref(oh)))
+        self.output_size = (ow.value, oh.value)  # (width, height), light.* padding included
         ctx._sessions.add(self)
 
     def close(self) -> None:

@@ -280,7 +280,8 @@ void smooth_create(const Map* in, Map* out, void*, Core* core, const Api* api) {
     f->vi = vi;
     if (svpf_smooth_output_rate(f->session, &f->vi.fps_num, &f->vi.fps_den) != SVPF_OK)
         return fail(svpf_last_error());
-    f->vi.num_frames = nout;  // output_info (core.rs:603-613); non-zero light padding is rejected at create
+    f->vi.num_frames = nout;  // output_info (core.rs:603-613), widened by the light.* padding
+    if (svpf_smooth_output_size(f->session, &f->vi.width, &f->vi.height) != SVPF_OK) return fail(svpf_last_error());
     create(in, out, "SVSmoothFps", smooth_init, smooth_get_frame, smooth_free, kFmParallel, 0, f, core);
 }
 

@@ -312,11 +312,85 @@ int oracle_smooth_render(const char* opt, size_t len, int width, int height, int
     c.super_rows = super_rows;
     c.vec_len = vec_len;
     c.float_render = float_render != 0;
-    uint8_t* dst[3] = {dy, du, dv};
+    int px = 0, py = 0;
+    light_padding(c.opt.light, width, height, &px, &py);
     int algo = 0;
-    int r = smooth_render_frame(c, k, dst, &algo);
+    if (px == 0 && py == 0) {
+        uint8_t* dst[3] = {dy, du, dv};
+        int r = smooth_render_frame(c, k, dst, &algo);
+        *algo_out = algo;
+        return r;
+    }
+    // padded output (core.rs:560-613, 634-669, 1563-1590): the frame rendered (or
+    // copied) at (pad_x, pad_y), then the light border of output frame k around it
+    std::vector<uint8_t> ty((size_t)width * height), tu((size_t)(width / 2) * (height / 2)), tv(tu.size());
+    uint8_t* tmp[3] = {ty.data(), tu.data(), tv.data()};
+    int r = smooth_render_frame(c, k, tmp, &algo);
     *algo_out = algo;
+    const int ow = width + 2 * px, oh = height + 2 * py;
+    uint8_t* out[3] = {dy, du, dv};
+    const int strides[3] = {ow, ow / 2, ow / 2}, rows[3] = {oh, oh / 2, oh / 2};
+    for (int p = 0; p < 3; p++) {
+        const int w = p ? width / 2 : width, h = p ? height / 2 : height;
+        const int ox = p ? px / 2 : px, oy = p ? py / 2 : py;
+        for (int y = 0; y < h; y++)
+            std::memcpy(out[p] + (size_t)(y + oy) * strides[p] + ox, tmp[p] + (size_t)y * w, (size_t)w);
+    }
+    light_apply(width, height, px, py, c.opt.light_params, k, out, strides, rows);
     return r;
+}
+
+// Options::padding of an option string (0 ok, -3 bad options).
+int oracle_smooth_padding(const char* opt, size_t len, int width, int height, int64_t fps_num, int64_t fps_den,
+                          const int32_t* vdata16, int32_t* pad_x, int32_t* pad_y) {
+    Value v;
+    bool have = parse_opt(opt, len, &v);
+    VectorData vd;
+    if (!parse_vdata(vdata16, &vd)) return -1;
+    SmoothOptions o;
+    std::string e;
+    if (!smooth_options(have ? &v : nullptr, fps_num, fps_den, vd, &o, &e)) return -3;
+    light_padding(o.light, width, height, pad_x, pad_y);
+    return 0;
+}
+
+// The light border's parts, for the tests that pin them (light.rs).
+void oracle_light_rng(uint32_t seed, int n, int32_t* out) {
+    uint32_t st = seed;
+    for (int i = 0; i < n; i++) {
+        st = st * 214013u + 2531011u;
+        out[i] = (int32_t)((st >> 16) & 0x7fff);
+    }
+}
+void oracle_light_profile(const int64_t* accum, int dim, int border, int lights, double cell, int base, float* out) {
+    std::vector<int64_t> a(accum, accum + std::max(dim, 0));
+    std::vector<float> r = light_profile(a, dim, border, lights, cell, base);
+    std::copy(r.begin(), r.end(), out);
+}
+void oracle_light_cubic(const int32_t* control, int lights, int dim, float* out) {
+    std::vector<int> c(control, control + lights);
+    std::vector<float> r = light_cubic_upsample(c, dim, lights);
+    std::copy(r.begin(), r.end(), out);
+}
+void oracle_light_corner_geometry(int width, int height, int length, double* mix, double* weight) {
+    std::vector<double> m, w;
+    light_corner_geometry(width, height, length, &m, &w);
+    std::copy(m.begin(), m.end(), mix);
+    std::copy(w.begin(), w.end(), weight);
+}
+// The border of output frame `frame` around an interior already in the padded planes
+// (strides width + 2 pad_x, half for chroma).
+void oracle_light_apply(int width, int height, int pad_x, int pad_y, int border, int lights, int length, double cell,
+                        int frame, uint8_t* y, uint8_t* u, uint8_t* v) {
+    LightParams lp;
+    lp.border = border;
+    lp.lights = std::max(lights, 1);
+    lp.length = length;
+    lp.cell = std::max(cell, 0.1);
+    const int ow = width + 2 * pad_x, oh = height + 2 * pad_y;
+    uint8_t* planes[3] = {y, u, v};
+    const int strides[3] = {ow, ow / 2, ow / 2}, rows[3] = {oh, oh / 2, oh / 2};
+    light_apply(width, height, pad_x, pad_y, lp, frame, planes, strides, rows);
 }
 
 }  // extern "C"

@@ -25,7 +25,11 @@
 //     shift and fwd = -bwd (test_oracle_pins.py);
 //   - TruncDiv, global predictor, rescale_scores, lambdas, luma_byte, scene classes,
 //     coverage and magnitude masks as closed forms (test_oracle_closed_forms.py);
-//   - the render: identical sources render the source for every algorithm.
+//   - the render: identical sources render the source for every algorithm;
+//   - the light border and padding (light.cpp): the published srand/rand sequence,
+//     the textbook Catmull-Rom curve, brute-force window means, the corner closed
+//     forms, worked padding sizes and a numpy restatement of the strip and corner
+//     layout (test_oracle_light.py).
 // One rule of the reference has no observable effect and so no check: the lsad/3 of
 // the coarsest level, where the lambda it scales is always 0.
 #pragma once

@@ -136,6 +136,15 @@ struct Timing {  // frame_math.rs:1-136
 
 // ---------------------------------------------------------------- SmoothFps
 
+// Light::from_value (options.rs:147-156) and LightParams (options.rs:279-286).
+struct LightOpts {
+    double sar = 1.0, aspect = 0.0, zoom = 0.0;
+};
+struct LightParams {
+    int border = 12, lights = 16, length = 100;
+    double cell = 1.0;
+};
+
 struct SmoothOptions {  // the options.rs subset the render path reads
     bool rate_abs = false;
     bool has_num = false, has_den = false;
@@ -151,6 +160,8 @@ struct SmoothOptions {  // the options.rs subset the render path reads
     bool mask_area_enabled = false;
     double area_scale = 1.0, area_sharp = 1.0;
     bool block = false;
+    LightOpts light;
+    LightParams light_params;
     Timing timing(int64_t snum, int64_t sden) const;
 };
 // Options::from_value + create-time adjustments (options.rs:130-240, filter.rs:44-139).
@@ -184,5 +195,16 @@ int scene_class(const std::vector<DVec>* prev, const std::vector<DVec>* cur, con
 // Renders output frame k into dst (3 planes, strides width, width/2).  Returns a code of
 // what happened: 0 copied source n, 1 copied source n+1, 2 rendered; *algo_out = algorithm used.
 int smooth_render_frame(const SmoothClip& c, int k, uint8_t* dst[3], int* algo_out);
+
+// Light border and output padding (light.rs, options.rs:256-277, 788-796).
+void light_padding(const LightOpts& l, int width, int height, int* pad_x, int* pad_y);
+void light_apply(int width, int height, int pad_x, int pad_y, const LightParams& lp, int frame, uint8_t* planes[3],
+                 const int strides[3], const int rows[3]);
+std::vector<int> light_control_points(const std::vector<int64_t>& accum, int dim, int lights, int border, int half,
+                                      int window, int base);
+std::vector<float> light_cubic_upsample(const std::vector<int>& control, int dim, int lights);
+std::vector<float> light_profile(const std::vector<int64_t>& accum, int dim, int border, int lights, double cell,
+                                 int base);
+void light_corner_geometry(int width, int height, int length, std::vector<double>* mix, std::vector<double>* weight);
 
 }  // namespace oracle

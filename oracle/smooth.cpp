@@ -2,8 +2,9 @@
 // crates/svpflow2/src/options.rs (parsing, create-time adjustments), core.rs
 // (source_frame/phase_256 :2054-2077, try_render :752-883, build_frame_prep
 // :1095-1260, render_into_output :1262-1770) and metadata.rs scene classification
-// (:275-362, 523-526).  Scene blending, HDR/DoVi, light padding and the debug
-// overlays are not part of this path.  Test infrastructure only (see oracle.h).
+// (:275-362, 523-526).  The light border of a padded output is drawn by light.cpp
+// after the interior is rendered; scene blending, HDR/DoVi and the debug overlays are
+// not part of this path.  Test infrastructure only (see oracle.h).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -137,6 +138,15 @@ bool smooth_options(const Value* opt, int64_t fps_num, int64_t fps_den, const Ve
     o->lim_m1 = I({"scene", "limits", "m1"}, &iv) ? iv : 1600;
     o->lim_m2 = I({"scene", "limits", "m2"}, &iv) ? iv : 2800;
     o->lim_scene = I({"scene", "limits", "scene"}, &iv) ? iv : 4000;
+    // Light::from_value (options.rs:147-156), Options::light_params (:279-286)
+    o->light.sar = F({"light", "sar"}, &fv) ? fv : 1.0;
+    o->light.aspect = F({"light", "aspect"}, &fv) ? fv : 0.0;
+    o->light.zoom = F({"light", "zoom"}, &fv) ? fv : 0.0;
+    auto sat32 = [](int64_t v) { return (int)std::min<int64_t>(std::max<int64_t>(v, INT32_MIN), INT32_MAX); };
+    o->light_params.border = sat32(I({"light", "border"}, &iv) ? iv : 12);
+    o->light_params.lights = std::max(sat32(I({"light", "lights"}, &iv) ? iv : 16), 1);
+    o->light_params.length = sat32(I({"light", "length"}, &iv) ? iv : 100);
+    o->light_params.cell = std::max(F({"light", "cell"}, &fv) ? fv : 1.0, 0.1);
     o->mask_area = I({"mask", "area"}, &iv) ? iv : 0;  // Mask::from_value, options.rs:569-581
     o->mask_cover = std::max<int64_t>(I({"mask", "cover"}, &iv) ? iv : 100, 0);
     o->mask_area_enabled = o->mask_area > 0;

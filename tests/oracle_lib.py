@@ -216,10 +216,21 @@ class SmoothClip:
         self.vec = ptrs(payloads)
         self.vec_len = len(payloads[0])
 
+    def padding(self):
+        """Options::padding (options.rs:256-277): (pad_x, pad_y) on each side."""
+        px, py = C.c_int32(), C.c_int32()
+        rc = L.oracle_smooth_padding(self.opt, len(self.opt), self.w, self.h, self.fps[0], self.fps[1], self.vdata,
+                                     C.byref(px), C.byref(py))
+        assert rc == 0, rc
+        return px.value, py.value
+
     def render(self, k):
-        Y = np.zeros((self.h, self.w), np.uint8)
-        U = np.zeros((self.h // 2, self.w // 2), np.uint8)
-        V = np.zeros((self.h // 2, self.w // 2), np.uint8)
+        """Output frame k (padded by the light options' border, if any)."""
+        px, py = self.padding()
+        ow, oh = self.w + 2 * px, self.h + 2 * py
+        Y = np.zeros((oh, ow), np.uint8)
+        U = np.zeros((oh // 2, ow // 2), np.uint8)
+        V = np.zeros((oh // 2, ow // 2), np.uint8)
         algo = C.c_int32()
         sup = [s if s is not None else C.cast(None, _pp) for s in self.sup]
         rc = L.oracle_smooth_render(self.opt, len(self.opt), self.w, self.h, self.fps[0], self.fps[1], self.n,
@@ -228,6 +239,52 @@ class SmoothClip:
                                     C.byref(algo), self.float_render)
         assert rc >= 0, rc
         return (Y, U, V), rc, algo.value
+
+L.oracle_smooth_padding.argtypes = [C.c_char_p, _sz, _i, _i, C.c_int64, C.c_int64, C.POINTER(C.c_int32),
+                                    C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+L.oracle_light_rng.argtypes = [C.c_uint32, _i, C.POINTER(C.c_int32)]
+L.oracle_light_profile.argtypes = [C.POINTER(C.c_int64), _i, _i, _i, C.c_double, _i, C.POINTER(C.c_float)]
+L.oracle_light_cubic.argtypes = [C.POINTER(C.c_int32), _i, _i, C.POINTER(C.c_float)]
+L.oracle_light_corner_geometry.argtypes = [_i, _i, _i, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+L.oracle_light_apply.argtypes = [_i, _i, _i, _i, _i, _i, _i, C.c_double, _i, u8p, u8p, u8p]
+
+
+def light_rng(seed: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.int32)
+    L.oracle_light_rng(seed & 0xFFFFFFFF, n, out.ctypes.data_as(C.POINTER(C.c_int32)))
+    return out
+
+
+def light_profile(accum, border: int, lights: int, cell: float, base: int) -> np.ndarray:
+    a = np.ascontiguousarray(accum, np.int64)
+    out = np.zeros(a.size, np.float32)
+    L.oracle_light_profile(a.ctypes.data_as(C.POINTER(C.c_int64)), a.size, border, lights, cell, base,
+                           out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
+
+
+def light_cubic(control, dim: int) -> np.ndarray:
+    c = np.ascontiguousarray(control, np.int32)
+    out = np.zeros(dim, np.float32)
+    L.oracle_light_cubic(c.ctypes.data_as(C.POINTER(C.c_int32)), c.size, dim, out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
+
+
+def light_corner_geometry(w: int, h: int, length: int):
+    mix = np.zeros(w * h, np.float64)
+    wt = np.zeros(w * h, np.float64)
+    L.oracle_light_corner_geometry(w, h, length, mix.ctypes.data_as(C.POINTER(C.c_double)),
+                                   wt.ctypes.data_as(C.POINTER(C.c_double)))
+    return mix.reshape(h, w), wt.reshape(h, w)
+
+
+def light_apply(planes, width: int, height: int, pad_x: int, pad_y: int, frame: int, border=12, lights=16,
+                length=100, cell=1.0):
+    """Draws the border of output frame `frame` into padded planes (modified in place)."""
+    Y, U, V = planes
+    L.oracle_light_apply(width, height, pad_x, pad_y, border, lights, length, cell, frame, _p(Y), _p(U), _p(V))
+    return planes
+
 
 # Per-block trace of the oracle's level searches (BlockTrace in oracle/analyse.h, 64 int32 words).
 TRACE_FIELDS = ["level", "dir", "bx", "by", "px", "py", "bw", "bh", "edge", "smallest",

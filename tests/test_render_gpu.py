@@ -50,8 +50,8 @@ class Pipeline:
         self.host_sup = [d.download() for d in self.sup]
         self.host_pay = [ctx.download(p, plen) for p in self.pay]
 
-    def outputs(self, n):
-        w, h = self.w, self.h
+    def outputs(self, n, size=None):
+        w, h = size or (self.w, self.h)
         return [svpf.DeviceFrame.alloc(self.ctx, (w, w // 2, w // 2), (h, h // 2, h // 2)) for _ in range(n)]
 
 
@@ -63,7 +63,7 @@ def _check(ctx, oracle, pipe, opt, fps=(30, 1), per_frame=False, expect_render=T
     try:
         sm.set_sources(0, pipe.src, pipe.sup, pipe.pay)
         nout = sm.output_frames
-        outs = pipe.outputs(nout)
+        outs = pipe.outputs(nout, sm.output_size)
         if per_frame:
             dec = []
             for k in range(nout):
@@ -74,9 +74,12 @@ def _check(ctx, oracle, pipe, opt, fps=(30, 1), per_frame=False, expect_render=T
         clip = oracle.SmoothClip(opt, pipe.frames, pipe.host_sup, pipe.host_pay, pipe.sdata, pipe.vdata, fps=fps)
         assert nout == oracle.smooth_opts(opt, fps[0], fps[1], pipe.vdata, n)[0]["out_frames"]
         rendered = 0
+        px, py = clip.padding()
+        assert sm.output_size == (pipe.w + 2 * px, pipe.h + 2 * py)
         for k in range(nout):
             want, rc, algo = clip.render(k)
             got = outs[k].download()
+            assert got[0].shape == want[0].shape
             assert dec[k] == (rc, algo if rc == 2 else 0), f"frame {k}: decision {dec[k]} want {(rc, algo)}"
             rendered += rc == 2
             for p in range(3):
@@ -152,6 +155,28 @@ def test_render_cfg3_settings_small(gpu_ctx, oracle):
 ])
 def test_render_options(gpu_ctx, oracle, pipe1, opt):
     _check(gpu_ctx, oracle, pipe1, opt)
+
+
+@pytest.mark.parametrize("light", [
+    "light:{aspect:2.35}",                                   # left and right strips only
+    "light:{aspect:1.33}",                                   # top and bottom strips only
+    "light:{zoom:10}",                                       # all four strips and the corners
+    "light:{zoom:10,border:4,lights:5,length:20,cell:2.5}",  # chroma's 9-row strips unfaded
+    "light:{aspect:2.35,zoom:5,length:0,border:-3}",         # corners of weight 0, border at least 1
+    "light:{zoom:3,lights:1,cell:0.05}",                     # 8 x 6 pads, one control point
+    "light:{aspect:1.0,sar:0.5,lights:64,cell:40}",          # windows wider than the frame
+])
+def test_render_light_border(gpu_ctx, oracle, pipe1, light):
+    """light.* padding: the output is widened by Options::padding (options.rs:256-277), the
+    frame rendered (or copied) at the pad offset and the light border drawn around it on
+    every output frame (light.rs), as the oracle restates it."""
+    _check(gpu_ctx, oracle, pipe1, "{rate:{num:2,den:1},algo:13,scene:{mode:0}," + light + "}")
+
+
+def test_render_light_border_algo23_per_frame(gpu_ctx, oracle, pipe2):
+    for per_frame in (False, True):
+        _check(gpu_ctx, oracle, pipe2, "{rate:{num:5,den:2},algo:23,mask:{area:100},light:{zoom:6,aspect:2.0}}",
+               fps=(24, 1), per_frame=per_frame)
 
 
 def test_render_cfg2_magnitude_exponents(gpu_ctx, oracle, pipe2):

@@ -87,10 +87,13 @@ def test_create_checks_and_clip_info(core):
                     opt=b"{rate:{num:2,den:1},algo:7}")
     with pytest.raises(vsfake.VSError, match="SVSmoothFps: invalid vectors stream"):
         core.invoke("svp2", "SmoothFps", clip=src, super=sup, sdata=sdata, vectors=vec, opt=ROPT)
-    # light.* options that widen the output (Options::padding, options.rs:256-277) are
-    # rejected instead of rendering a frame of another size than the reference's
-    for light in (b"light:{aspect:2.35}", b"light:{zoom:10}", b"light:{aspect:1.0,sar:0.5}"):
-        with pytest.raises(vsfake.VSError, match="light.aspect/light.zoom padding is not supported"):
+    # light.* padding (Options::padding, options.rs:256-277) is drawn within stated limits;
+    # settings beyond them are rejected rather than drawn differently
+    for light, msg in ((b"light:{zoom:10,border:300}", "light.border above 255 is not supported"),
+                       (b"light:{zoom:10,border:97}", "light.border deeper than the frame"),
+                       (b"light:{zoom:10,lights:5000}", "light.lights above 4096 a side is not supported"),
+                       (b"light:{aspect:1.0,sar:0.5,cell:2000}", "light.cell above 1000 is not supported")):
+        with pytest.raises(vsfake.VSError, match=msg):
             core.invoke("svp2", "SmoothFps", clip=src, super=sup, sdata=sdata, vectors=vec, vdata=vdata_addr,
                         opt=b"{rate:{num:2,den:1}," + light + b"}")
     with pytest.raises(vsfake.VSError, match="SVSmoothFps: CUDA is not availabe"):
@@ -134,3 +137,33 @@ def test_plugin_chain_matches_oracle(core, oracle):
                 f"output frame {k} plane {pl}"
         # every output lasts 1/60 s: (1/24 s) / 2.5, in units of 1e-7 s (frame.rs:111-138)
         assert got.props["_DurationNum"][0][1] == 10_000_000 // 60 and got.props["_DurationDen"][0][1] == 10_000_000
+
+
+@pytest.mark.gpu
+def test_plugin_light_padding_matches_oracle(core, oracle):
+    """SmoothFps with light.* padding: the output clip is widened (output_info, core.rs
+    :603-613) and every frame carries the light border (light.rs), as the oracle draws it."""
+    frames, src = _clip(core, n=4)
+    print("vs plugins: 160x96 with light.zoom padding", flush=True)
+    out = core.invoke("svp1", "Super", clip=src, opt=b"{pel:2}")
+    sup, sdata = out["clip"][0][1], out["data"][0][1]
+    a = core.invoke("svp1", "Analyse", clip=sup, sdata=sdata, src=src, opt=AOPT)
+    vec, vdata_addr = a["clip"][0][1], a["data"][0][1]
+    ropt = b"{rate:{num:5,den:2},algo:23,mask:{area:100},scene:{mode:0},light:{zoom:12,aspect:2.0}}"
+    s = core.invoke("svp2", "SmoothFps", clip=src, super=sup, sdata=sdata, vectors=vec, vdata=vdata_addr, opt=ropt)
+    smooth = s["clip"][0][1]
+    st = tuple(core.frame(sup, 0).planes[pl].shape[1] for pl in range(3))
+    hs = [oracle.super_build("{pel:2}", f, strides=st, dst_strides=st) for f in frames]
+    so, _ = oracle.super_opts("{pel:2}", 160, 96)
+    pays = [oracle.analyse(AOPT.decode(), so["sdata"], 160, 96, hs[i], hs[min(i + 1, 3)]) for i in range(4)]
+    ao, _ = oracle.analyse_opts(AOPT.decode(), so["sdata"], 160, 96)
+    clip = oracle.SmoothClip(ropt.decode(), frames, hs, pays, so["sdata"], ao["vdata"], fps=(24, 1))
+    px, py = clip.padding()
+    assert px > 0 and py > 0
+    svi = core.video_info(smooth)
+    assert (svi.width, svi.height) == (160 + 2 * px, 96 + 2 * py)
+    for k in range(svi.num_frames):
+        got = core.frame(smooth, k)
+        want, _, _ = clip.render(k)
+        for pl in range(3):
+            assert np.array_equal(got.view(pl), want[pl]), f"output frame {k} plane {pl}"
