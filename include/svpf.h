@@ -164,9 +164,9 @@ int svpf_analyse_set_max_sweeps(svpf_ctx* ctx, int32_t max_sweeps);
  *   coarse_slots sweep-0 slots of the coarser levels (default 256: one segment per chain
  *                from 256 chains on; their visit order is sorted by parent score, so a
  *                block's earlier neighbours lie anywhere before it);
- *   sweep_grid   workgroups of a later sweep (default 4096);
+ *   sweep_grid   workgroups of a later sweep (default 2048);
  *   fine_run     consecutive visit positions a finest-level sweep-0 item searches in
- *                order (1, 2, 4, 8, 16 or 32; default 1);
+ *                order (1, 2, 4, 8, 16 or 32; default 16);
  *   coarse_run   the same at the coarser levels (4, 8, 16 or 32; default 16);
  *   graphs       1 (default): the later sweeps of a level are replayed from a captured
  *                hipGraph of six sweeps (cached per launch configuration); -1: launched
@@ -209,7 +209,7 @@ typedef struct svpf_smooth svpf_smooth;
 int svpf_smooth_create(svpf_ctx* ctx, const char* opt, size_t opt_len, int32_t width, int32_t height, int64_t fps_num,
                        int64_t fps_den, int32_t nframes, int64_t sdata, const int32_t* vdata16, svpf_smooth** out,
                        int32_t* out_frames, char* err, size_t err_len);
-void svpf_smooth_destroy(svpf_smooth* s);
+void svpf_smooth_destroy(svpf_smooth* s);  /* before svpf_destroy of its context */
 /* Binds source frames first..first+count-1: the source planes, their Super frames
  * (d_super may be NULL when the vectors are pel 1) and the device vector payloads
  * written by svpf_analyse_* (the frame's getFrame inputs, core.rs:962-1051).  All

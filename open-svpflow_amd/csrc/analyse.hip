@@ -1691,8 +1691,10 @@ __global__ void __launch_bounds__(kWave, 2) sweep_kernel(const Chain* __restrict
             if (!sweep_item<false, kL0>(chains, ci, idx, st, s, &me) || k >= chase) break;
             const int next = order_successor(chains, ci, idx, me);
             if (next < 0 || !claim_block(st, base, next, s)) break;
-            // this wave's store of idx before the successor reads it
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            // this wave's store of idx before the successor reads it (the same wave: a
+            // workgroup-scope fence orders it; waves elsewhere that read the old value are
+            // queued again by idx's change)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             if (lane_id() == 0) atomicAdd(&st.dcount[kDcChased], 1u);  // chased (statistics)
             idx = next;
         }

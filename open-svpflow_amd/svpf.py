@@ -9,6 +9,7 @@ the library reports an error.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import weakref
@@ -97,6 +98,18 @@ def exported_symbols() -> list[str]:
     return sorted(set(re.findall(r"\b(svpf_[a-z0-9_]+)\s*\(", text)))
 
 
+_LIVE_CONTEXTS: "weakref.WeakSet[Context]" = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_contexts() -> None:
+    """Ends every context still open when the interpreter exits (render sessions first),
+    while the interpreter and the HIP runtime are whole: interpreter teardown would
+    otherwise finalise sessions and contexts in any order."""
+    for ctx in list(_LIVE_CONTEXTS):
+        ctx.close()
+
+
 class Context:
     """A libsvpf context: one device, one HIP stream."""
 
@@ -106,6 +119,7 @@ class Context:
         self.h = h
         self._allocs: list[int] = []
         self._sessions = weakref.WeakSet()  # render sessions to end before the context
+        _LIVE_CONTEXTS.add(self)
 
     def close(self) -> None:
         if self.h:
@@ -116,6 +130,7 @@ class Context:
             self._allocs.clear()
             lib().svpf_destroy(self.h)
             self.h = None
+        _LIVE_CONTEXTS.discard(self)
 
     def __enter__(self):
         return self
@@ -371,9 +386,12 @@ class Smooth:
         ctx._sessions.add(self)
 
     def close(self) -> None:
-        if self.h:
+        # a session is ended before its context (Context.close does so); a session met
+        # after its context closed (interpreter teardown) is only dropped, since its
+        # destroy call would use the context's stream
+        if self.h and self.ctx.h:
             lib().svpf_smooth_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
