@@ -166,7 +166,7 @@ int svpf_analyse_set_max_sweeps(svpf_ctx* ctx, int32_t max_sweeps);
  *                block's earlier neighbours lie anywhere before it);
  *   sweep_grid   workgroups of a later sweep (default 2048);
  *   fine_run     consecutive visit positions a finest-level sweep-0 item searches in
- *                order (1, 2, 4, 8, 16 or 32; default 16);
+ *                order (1, 2, 4, 8, 16 or 32; default 32);
  *   coarse_run   the same at the coarser levels (4, 8, 16 or 32; default 16);
  *   graphs       1 (default): the later sweeps of a level are replayed from a captured
  *                hipGraph of six sweeps (cached per launch configuration); -1: launched

@@ -38,7 +38,7 @@ SPACE: Dict[str, Tuple] = {
     "fine_run": (1, 2, 4, 8, 16, 32),
     "coarse_run": (4, 8, 16, 32),
 }
-DEFAULT: Dict[str, float] = {"slot_scale": 1.0, "sweep_grid": 2048, "coarse_slots": 256, "fine_run": 16, "coarse_run": 16}
+DEFAULT: Dict[str, float] = {"slot_scale": 1.0, "sweep_grid": 2048, "coarse_slots": 256, "fine_run": 32, "coarse_run": 16}
 ORDER = ("slot_scale", "fine_run", "sweep_grid", "coarse_slots", "coarse_run")
 
 

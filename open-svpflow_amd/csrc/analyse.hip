@@ -159,6 +159,38 @@ __device__ __forceinline__ uint32_t group_sum(uint32_t v) {
     return v;
 }
 
+// Half of a 4x4 tile's SATD, computed by a pair of lanes (quad lanes 2j and 2j + 1, both
+// active): this lane holds rows 0-1 (`upper`) or rows 2-3 of the tile.  The horizontal
+// butterflies and the vertical one within the two rows run in-lane; the vertical stage
+// joining the halves takes the partner's row sums (upper) or row differences (lower)
+// by DPP, so each lane completes half of the coefficients, and the last stage is
+// had4_rows_pk's 2 max(|a|, |b|).  The pair's two results add up to
+// had4_rows_pk(tile) / 2.
+__device__ __forceinline__ uint32_t had4_half_pk(const uint32_t a[2], const uint32_t b[2], bool upper) {
+    s16x2 P[2], M[2];
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const s16x2 A = as_s16x2(pk_bytes01(a[r])) - as_s16x2(pk_bytes01(b[r]));  // columns 0, 1
+        const s16x2 B = as_s16x2(pk_bytes23(a[r])) - as_s16x2(pk_bytes23(b[r]));  // columns 2, 3
+        P[r] = A + B;
+        M[r] = A - B;
+    }
+    const s16x2 ps = P[0] + P[1], pd = P[0] - P[1], ms = M[0] + M[1], md = M[0] - M[1];
+    // the upper lane keeps its sums and sends its differences, the lower one the reverse
+    const s16x2 rp = as_s16x2(dpp<kDppQuadXor1>(__builtin_bit_cast(uint32_t, upper ? pd : ps)));
+    const s16x2 rm = as_s16x2(dpp<kDppQuadXor1>(__builtin_bit_cast(uint32_t, upper ? md : ms)));
+    const s16x2 ap = upper ? ps : rp, bp = upper ? rp : pd;  // (rows 01 + rows 23) or (rows 01 - rows 23)
+    const s16x2 am = upper ? ms : rm, bm = upper ? rm : md;
+    const s16x2 q[4] = {ap + bp, ap - bp, am + bm, am - bm};
+    s16x2 acc = {0, 0};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const s16x2 v = __builtin_elementwise_max(q[k], -q[k]);
+        acc += __builtin_elementwise_max(v, v.yx);
+    }
+    return (uint32_t)(uint16_t)acc.x;
+}
+
 // One candidate's cost computed by the 4 lanes of a group (part = lane & 3); every
 // lane of the group receives (L, C).  Blocks whose luma and chroma lie inside their
 // levels (the common case) split into units -- 4x4 tiles for SATD, rows for SAD --
@@ -262,8 +294,9 @@ typedef __attribute__((address_space(1))) const uint32_t __attribute__((aligned(
 __device__ __forceinline__ uint32_t gld32(const g_u8* p) { return *reinterpret_cast<const g_u32u*>(p); }
 
 // Each lane of a group (part = lane % 4) owns the luma 4x4 tile `part` of an 8x8 block
-// (part 0 alone for 4x4); parts 0 and 1 also own the u and v tiles.  Their source rows
-// are loaded once per block.  Candidates whose luma and chroma blocks lie inside their
+// (part 0 alone for 4x4) and two rows of a chroma tile: rows 0-1 (even parts) or 2-3
+// (odd parts) of the u tile (parts 0-1) or the v tile (parts 2-3), whose SATD a pair of
+// lanes computes together (had4_half_pk).  Their source rows are loaded once per block.  Candidates whose luma and chroma blocks lie inside their
 // levels and off the edge rule (the vectors of one rectangle, fixed per block: the
 // conditions of group_cost's interior path) take this path; the rest take group_cost.
 struct TilePlan {
@@ -274,9 +307,9 @@ struct TilePlan {
     int lrs, satd;                // uniform: its stride; SATD (else SAD)
     int dc, n;                    // uniform: the tiles are the block's own pixels; block size
     int lrow0, lcol0, luse;       // lane: luma tile
-    int crow0, ccol0, cuse, crs;  // lane: chroma tile
+    int crow0, ccol0, cuse, crs;  // lane: two rows of a chroma tile (u: parts 0-1, v: parts 2-3)
     const g_u8* cbase;
-    uint32_t lsrc[4], csrc[4];
+    uint32_t lsrc[4], csrc[2];
 };
 
 __device__ __forceinline__ void make_plan(const Blk& b, bool chroma, int part, TilePlan& P) {
@@ -321,19 +354,19 @@ __device__ __forceinline__ void make_plan(const Blk& b, bool chroma, int part, T
     P.luse = N == 8 || part == 0;
     P.lrow0 = b.y_off + ipy + 4 * ty;
     P.lcol0 = ipx + 4 * tx;
-    P.cuse = cuse && part < 2;
-    P.crow0 = b.c_off + cpy;
+    P.cuse = cuse;
+    const int chalf = 2 * (part & 1);  // this lane's first row of its chroma tile
+    P.crow0 = b.c_off + cpy + chalf;
     P.ccol0 = cpx;
-    P.crs = part == 0 ? b.rs1 : b.rs2;
-    P.cbase = (const g_u8*)(part == 0 ? b.rp1 : b.rp2);
-    const int css = part == 0 ? b.ss1 : b.ss2;
+    P.crs = part < 2 ? b.rs1 : b.rs2;
+    P.cbase = (const g_u8*)(part < 2 ? b.rp1 : b.rp2);
+    const int css = part < 2 ? b.ss1 : b.ss2;
     const g_u8* ls = (const g_u8*)b.sp0 + (P.lrow0 * b.ss0 + P.lcol0);
-    const g_u8* cs = (const g_u8*)(part == 0 ? b.sp1 : b.sp2) + (P.crow0 * css + P.ccol0);
+    const g_u8* cs = (const g_u8*)(part < 2 ? b.sp1 : b.sp2) + (P.crow0 * css + P.ccol0);
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        P.lsrc[k] = P.luse ? gld32(ls + k * b.ss0) : 0u;
-        P.csrc[k] = P.cuse ? gld32(cs + k * css) : 0u;
-    }
+    for (int k = 0; k < 4; k++) P.lsrc[k] = P.luse ? gld32(ls + k * b.ss0) : 0u;
+#pragma unroll
+    for (int k = 0; k < 2; k++) P.csrc[k] = P.cuse ? gld32(cs + k * css) : 0u;
 }
 
 __device__ __forceinline__ bool plan_covers(const TilePlan& P, int mvx, int mvy) {
@@ -362,10 +395,17 @@ __device__ __forceinline__ void plan_cost(const TilePlan& P, int mvx, int mvy, u
         const int row = P.lrow0 + sub * P.lh + (mvy >> sh), col = P.lcol0 + (mvx >> sh);
         hl = tile_cost(P.lsrc, P.lbase + (row * P.lrs + col), P.lrs, P.satd != 0);
     }
-    if (P.cuse) {
+    if (P.cuse) {  // (uniform: the whole group takes part in the pair exchange)
         const int csh = P.subpel + 1;
         const int row = P.crow0 + sub * P.clh + (mvy >> csh), col = P.ccol0 + (mvx >> csh);
-        hc = tile_cost(P.csrc, P.cbase + (row * P.crs + col), P.crs, P.satd != 0);
+        const g_u8* r = P.cbase + (row * P.crs + col);
+        const uint32_t y[2] = {gld32(r), gld32(r + P.crs)};
+        if (P.satd) {
+            hc = had4_half_pk(P.csrc, y, (threadIdx.x & 1) == 0);
+        } else {
+            hc = __builtin_amdgcn_sad_u8(P.csrc[0], y[0], 0u);
+            hc = __builtin_amdgcn_sad_u8(P.csrc[1], y[1], hc);
+        }
     }
     L = group_sum(hl);
     C = 4u * group_sum(hc);

@@ -966,7 +966,7 @@ int svpf_analyse_set_tuning(svpf_ctx* ctx, const svpf_analyse_tuning* t) {
         return fail(SVPF_ERR_ARG, "svpf_analyse_set_tuning: setting out of range");
     ctx->chase = std::max(t->chase, 0);
     ctx->slot_scale = t->slot_scale == 0.0 ? 1.0 : t->slot_scale;
-    ctx->fine_run = t->fine_run == 0 ? 16 : t->fine_run;
+    ctx->fine_run = t->fine_run == 0 ? 32 : t->fine_run;
     ctx->coarse_slots = t->coarse_slots == 0 ? 256 : t->coarse_slots;
     ctx->coarse_run = t->coarse_run == 0 ? 16 : t->coarse_run;
     ctx->sweep_grid = t->sweep_grid == 0 ? 2048 : t->sweep_grid;

@@ -32,7 +32,7 @@ struct svpf_ctx {
     // block changes (0: none, every change waits for the next sweep).
     int chase = 0;
     double slot_scale = 1.0;   // factor on the finest level's sweep-0 slot count
-    int fine_run = 16;         // finest-level positions per sweep-0 item
+    int fine_run = 32;         // finest-level positions per sweep-0 item
     int coarse_slots = 256;    // sweep-0 slots of the coarser levels
     int coarse_run = 16;       // coarser-level positions per sweep-0 item
     int sweep_grid = 2048;     // workgroups of a later sweep
