@@ -1409,13 +1409,20 @@ __global__ void sweep_init_kernel(const Chain* __restrict__ chains, SweepState s
         }
 }
 
-// Block n of chain ci into sweep s + 1's list wl[q][counts[(s + 1) % 3]++], once (flags[q]).
-__device__ __forceinline__ void queue_block(const SweepState& st, int ci, const Chain& ch, int n, int s) {
+// Block n of chain ci into sweep s + 1's list wl[q], once (flags[q]), for every calling
+// lane whose `want` is set.  The list length is one counter for the whole level, so the
+// lanes that win their block's flag take consecutive slots with one counter update per
+// wave (the same counter updated by every block would serialise the queueing).
+__device__ __forceinline__ void queue_wave(const SweepState& st, int ci, const Chain& ch, int n, bool want, int s) {
     const int q = (s + 1) & 1;
-    if (atomicExch(&st.flags[q][ch.base + n], 1u) == 0u) {
-        const uint32_t pos = atomicAdd(&st.counts[(s + 1) % 3], 1u);
-        st.wl[q][pos] = ((unsigned long long)ci << 32) | (unsigned)n;
-    }
+    const bool take = want && atomicExch(&st.flags[q][ch.base + n], 1u) == 0u;
+    const uint64_t m = __ballot(take);
+    if (m == 0) return;
+    const int leader = __builtin_amdgcn_readfirstlane(__builtin_ctzll(m));
+    uint32_t first = 0;
+    if (lane_id() == leader) first = atomicAdd(&st.counts[(s + 1) % 3], (uint32_t)__popcll(m));
+    first = (uint32_t)__builtin_amdgcn_readlane((int)first, leader);
+    if (take) st.wl[q][first + lane_rank(m)] = ((unsigned long long)ci << 32) | (unsigned)n;
 }
 
 constexpr int kScanThreads = 512;  // threads of a workgroup that rebuilds one chain's count
@@ -1501,7 +1508,7 @@ __device__ void scan_chain(const Chain* __restrict__ chains, int ci, const Sweep
         const int idx = (pk >> 16) * g.gw + (pk & 0xFFFF);
         const int bj = (int)fs[k] > sat_add_i32(g.bad_sad, sat_mul_i32(g.bad_sad, c) / 16) ? 1 : 0;
         st.cpre[ch.base + idx] = c;
-        if ((uint32_t)bj != st.badf[ch.base + idx]) queue_block(st, ci, ch, idx, s);
+        queue_wave(st, ci, ch, idx, (uint32_t)bj != st.badf[ch.base + idx], s);
         c += bj;
     }
     if (tid == 0) st.minpos[ci] = 0xFFFFFFFFu;
@@ -1567,14 +1574,11 @@ __device__ __forceinline__ bool sweep_item(const Chain* __restrict__ chains, int
     // sweep 0 queues nothing: validate_kernel compares every block's seed record with
     // its neighbours once the sweep is complete
     if (!moved || s == 0) return moved;
-    if (lane < 9 && lane != 4) {
-        const int bx = idx % g.gw, by = idx / g.gw;
-        const int nx = bx - 1 + lane % 3, ny = by - 1 + lane / 3;
-        if (nx >= 0 && ny >= 0 && nx < g.gw && ny < g.gh) {
-            const int n = ny * g.gw + nx;
-            if (ch.opos[n] > me) queue_block(st, ci, ch, n, s);
-        }
-    }
+    const int bx = idx % g.gw, by = idx / g.gw;
+    const int nx = bx - 1 + lane % 3, ny = by - 1 + lane / 3;
+    const bool in = lane < 9 && lane != 4 && nx >= 0 && ny >= 0 && nx < g.gw && ny < g.gh;
+    const int n = in ? ny * g.gw + nx : idx;
+    queue_wave(st, ci, ch, n, in && ch.opos[n] > me, s);
     return true;
 }
 
@@ -1645,30 +1649,38 @@ __global__ void __launch_bounds__(kWave, kFast ? 3 : 2) sweep0_kernel(const Chai
 // whose recorded neighbour vectors differ from the neighbours' results now (every
 // other input of a block off the edge row and column is fixed for the level; a changed
 // bad-block count is handled by the count rebuild), and every edge block (no record).
-// One thread per block, blocks in raster order.
-__global__ void validate_kernel(const Chain* __restrict__ chains, SweepState st, int max_n, long long total) {
+// One thread per block, blocks in raster order; a thread issues all its loads at once
+// (the record as three 16-byte words, each neighbour's visit position and vector).
+__global__ void __launch_bounds__(256) validate_kernel(const Chain* __restrict__ chains, SweepState st, int max_n,
+                                                       long long total) {
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
          t += (long long)gridDim.x * blockDim.x) {
-        const int ci = (int)(t / max_n), idx = (int)(t - (long long)ci * max_n);
+        const int ci = (int)(t / max_n), idx0 = (int)(t - (long long)ci * max_n);
         const Chain& ch = chains[ci];
         const LevelGeom& g = ch.geom;
         const int gw = g.gw, gh = g.gh;
-        if (idx >= gw * gh) continue;
+        const bool here = idx0 < gw * gh;
+        const int idx = here ? idx0 : 0;
         const int bx = idx % gw, by = idx / gw;
-        bool stale = g.level > 0 && (bx == gw - 1 || by == gh - 1);
-        if (!stale) {
-            const SeedRec& r = st.seeds[ch.base + idx];
-            stale = r.valid == 0;
-            const int me = ch.opos[idx];
-            for (int q = 0; q < 9 && !stale; q++) {
-                const int nx = bx - 1 + q % 3, ny = by - 1 + q / 3;
-                if (q == 4 || nx < 0 || ny < 0 || nx >= gw || ny >= gh) continue;
-                const int n = ny * gw + nx;
-                if (ch.opos[n] >= me) continue;
-                stale = *reinterpret_cast<const uint32_t*>(&ch.res[n]) != r.seed[q];
-            }
+        const bool edge = g.level > 0 && (bx == gw - 1 || by == gh - 1);
+        const uint4* rw = reinterpret_cast<const uint4*>(&st.seeds[ch.base + idx]);
+        const uint4 r0 = rw[0], r1 = rw[1], r2 = rw[2];
+        const uint32_t seed[9] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x};
+        const int me = ch.opos[idx];
+        int pos[9];
+        uint32_t vec[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            const int nx = bx - 1 + q % 3, ny = by - 1 + q / 3;
+            const bool in = q != 4 && nx >= 0 && ny >= 0 && nx < gw && ny < gh;
+            const int n = in ? ny * gw + nx : idx;
+            pos[q] = in ? ch.opos[n] : INT32_MAX;
+            vec[q] = *reinterpret_cast<const uint32_t*>(&ch.res[n]);
         }
-        if (stale) queue_block(st, ci, ch, idx, 0);
+        bool stale = edge || r2.z == 0;  // no record at the edge; SeedRec::valid
+#pragma unroll
+        for (int q = 0; q < 9; q++) stale = stale || (pos[q] < me && vec[q] != seed[q]);
+        queue_wave(st, ci, ch, idx, here && stale, 0);
     }
 }
 

@@ -1,6 +1,7 @@
 // Device-side data layouts of the Analyse search (shared by analyse.hip and the
 // host orchestration in analyse_host.cpp).
 #pragma once
+#include <cstddef>
 #include <cstdint>
 
 namespace svpf {
@@ -80,6 +81,8 @@ struct alignas(16) SeedRec {
     uint32_t valid;
     uint32_t pad;
 };
+// validate_kernel reads a record as three 16-byte words (valid: word 10)
+static_assert(sizeof(SeedRec) == 48 && offsetof(SeedRec, valid) == 40, "SeedRec layout");
 
 // Worklists of the parallel sweeps of one level (all chains together).  Sweep s
 // processes wl[s & 1][0 .. counts[s]) whose members are flagged in flags[s & 1]; a
