@@ -103,11 +103,19 @@ __global__ void prep_vectors_kernel(const PrepJob* __restrict__ jobs, const Smoo
         const int luma = max((int)g.luma_lut[a.luma + b.luma], 1);
         const long long s255 = (long long)b.score * 255;
         const int score = (int)(s255 > 2147483647LL ? 2147483647LL : s255) / luma;
-        if (score < g.lim_zero) continue;
-        atomicAdd(&j.stats[2], 1);
-        if (score >= g.lim_scene) atomicAdd(&j.stats[3], 1);
-        else if (score >= g.lim_m2) atomicAdd(&j.stats[4], 1);
-        else if (score >= g.lim_m1) atomicAdd(&j.stats[5], 1);
+        // the frame's class counts (classify_scene): one add per counter per wave, the
+        // wave's cells counted by ballot (every cell adding to one word would serialise)
+        const bool considered = score >= g.lim_zero;
+        const bool scene = considered && score >= g.lim_scene;
+        const bool m2 = considered && !scene && score >= g.lim_m2;
+        const bool m1 = considered && !scene && !m2 && score >= g.lim_m1;
+        const uint64_t bc = __ballot(considered), bs = __ballot(scene), b2 = __ballot(m2), b1 = __ballot(m1);
+        if (bc != 0 && (threadIdx.x & 63) == (unsigned)__builtin_ctzll(bc)) {
+            atomicAdd(&j.stats[2], (int)__popcll(bc));
+            if (bs) atomicAdd(&j.stats[3], (int)__popcll(bs));
+            if (b2) atomicAdd(&j.stats[4], (int)__popcll(b2));
+            if (b1) atomicAdd(&j.stats[5], (int)__popcll(b1));
+        }
     }
 }
 
