@@ -33,6 +33,7 @@
 //     score sort, child visit order) run as separate kernels over all chains, so a
 //     batch of pairs never returns to the host between levels.
 #include <cmath>
+#include <type_traits>
 #include <hipcub/hipcub.hpp>
 
 #include "analyse_cost.h"
@@ -564,17 +565,17 @@ __device__ __forceinline__ void pat_hex(int cx, int cy, unsigned e, int& x, int&
     y = cy + (int)((kHex2Y >> (4 * e)) & 15u) - 2;
 }
 
-__device__ __forceinline__ void pat_gen(const Pattern& p, int k, int& x, int& y) {
+__device__ __forceinline__ void pat_gen_kind(int kind, const Pattern& p, int k, int& x, int& y) {
     const int cx = p.cx, cy = p.cy, r = p.r;
     const unsigned uk = (unsigned)k;
-    if (p.kind == kPatExpand) {
+    if (kind == kPatExpand) {
         pat_expand(cx, cy, r, p.step, k, x, y);
-    } else if (p.kind == kPatHexFirst || p.kind == kPatHexStep) {  // kHex2[k + 1], or kHex2[odir + k] (p.step = odir)
-        pat_hex(cx, cy, p.kind == kPatHexFirst ? uk + 1 : (unsigned)p.step + uk, x, y);
-    } else if (p.kind == kPatHexExp) {
+    } else if (kind == kPatHexFirst || kind == kPatHexStep) {  // kHex2[k + 1], or kHex2[odir + k] (p.step = odir)
+        pat_hex(cx, cy, kind == kPatHexFirst ? uk + 1 : (unsigned)p.step + uk, x, y);
+    } else if (kind == kPatHexExp) {
         if (k < 6) pat_hex(cx, cy, uk + 1, x, y);
         else pat_expand(cx, cy, 1, 1, k - 6, x, y);
-    } else if (p.kind == kPatUmh) {
+    } else if (kind == kPatUmh) {
         const int nd = r / 2;
         if (k < 2 * nd) {
             const int d = 1 + 2 * (int)(uk >> 1);
@@ -600,6 +601,7 @@ __device__ __forceinline__ void pat_gen(const Pattern& p, int k, int& x, int& y)
         y = cy - r + (int)q;
     }
 }
+__device__ __forceinline__ void pat_gen(const Pattern& p, int k, int& x, int& y) { pat_gen_kind(p.kind, p, k, x, y); }
 
 // adapt_lambda (:1555-1564), IEEE double as the reference.
 __device__ __attribute__((always_inline)) int adapt_lambda(int nl, int lsad, uint32_t score) {
@@ -966,20 +968,25 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
     if (edge) memo = nullptr;
     bool memo_hit = false, provisional = false;
     int key_x = 0, key_y = 0;
-    for (;;) {
-        // ---------------- the evaluation site: kPerPass candidates per pass, one group each
+    // ---------------- the evaluation site: the current list P, kPerPass candidates per
+    // pass, one group each; returns the index of the list's winner (-1: none improved).
+    // Kind is the list's kind as a compile-time constant (its branches fold), or -1 to
+    // read it from P.
+    auto eval_site = [&](auto kind_c) -> int {
+        constexpr int kKind = decltype(kind_c)::value;
         int win = -1;
         {
+            const int kind = kKind >= 0 ? kKind : (int)uniform_int(P.kind);
             // the pattern is wave-uniform: keep it in scalar registers so the pattern
             // branches below are scalar branches
             P = uniform_load(&P);
-            const bool list = P.kind == kPatPred;
+            const bool list = kind == kPatPred;
             // Long lists (UMH, exhaustive, wide rings) go in windows of 2 * kWave points:
             // one lane per point drops the points out of bounds or whose motion penalty
             // already reaches the best cost, and the survivors, in list order, fill the
             // passes densely.  The pass index below is then a survivor's rank, which
             // orders them as their list index does.
-            const bool dense = !list && !cur.serial && P.n > kPerPass && P.kind != kPatHexExp;
+            const bool dense = !list && !cur.serial && P.n > kPerPass && kind != kPatHexExp;
             const int wlen = dense ? 2 * kWave : max(P.n, 1);
             for (int w0 = 0; w0 < P.n; w0 += wlen) {
                 int cnt = P.n, n0 = 0;
@@ -988,11 +995,11 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                     bool a0 = false, a1 = false;
                     int x, y;
                     if (w0 + lane < P.n) {
-                        pat_gen(P, w0 + lane, x, y);
+                        pat_gen_kind(kind, P, w0 + lane, x, y);
                         a0 = survives(cur, best.cost, x, y);
                     }
                     if (w0 + kWave + lane < P.n) {
-                        pat_gen(P, w0 + kWave + lane, x, y);
+                        pat_gen_kind(kind, P, w0 + kWave + lane, x, y);
                         a1 = survives(cur, best.cost, x, y);
                     }
                     m0 = __ballot(a0);
@@ -1025,13 +1032,13 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                     const int i0 = fwd_permute(d0, w0 + lane), i1 = fwd_permute(d1, w0 + kWave + lane);
                     const int li = (int)quad_bcast0((uint32_t)(k < n0 ? i0 : i1));
                     if (valid) {
-                        pat_gen(P, li, mx, my);
+                        pat_gen_kind(kind, P, li, mx, my);
                         // (inside the bounds) the best may have improved in an earlier pass
                         motion = motion_of(cur, mx, my);
                         valid = motion < best.cost;
                     }
                 } else if (valid) {
-                    pat_gen(P, k, mx, my);
+                    pat_gen_kind(kind, P, k, mx, my);
                     valid = in_bounds(cur, mx, my);
                     if (valid) {
                         motion = motion_of(cur, mx, my);
@@ -1072,7 +1079,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                     // lowest (cost, index) == the reference's running strict-`<` minimum;
                     // kPatHexExp: the hexagon's 6 points first, and the ring only when none
                     // of them improves (exact_hex2 then expands around the same centre)
-                    const bool two = P.kind == kPatHexExp;
+                    const bool two = kind == kPatHexExp;
                     const bool in1 = valid && (!two || k < 6);
                     int64_t mc;
                     int mk;
@@ -1126,6 +1133,94 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
             }
             }
         }
+        return win;
+    };
+    if constexpr (kL0 && kFast) {
+        // The finest level's steps as straight-line code (the step machine below with its
+        // transitions fixed: predictor list, then UMH's lists and hex2's, hex2's alone or
+        // the exhaustive window, then the memo), each list evaluated by the evaluation
+        // site specialised for its kind.  No edge row or column, no bad-block pass and no
+        // serial replay at this level; the tile-plan instantiation only (the general cost
+        // path, specialised five times, would multiply the later sweeps' code).
+        using std::integral_constant;
+        (void)eval_site(integral_constant<int, kPatPred>{});
+        cur.lambda = adapt_lambda(g.lambda_base, g.lsad, pred.score);
+        if (g.early) {  // distance 0: the predictor (or zero) is the answer
+            const int src = field_valid ? 0 : 3;
+            best.mvx = rl(lx, src);
+            best.mvy = rl(ly, src);
+            best.sad = rlu(lsad, src);
+            best.cost = (int64_t)best.sad;
+            found = best.sad;
+        } else {
+            if (has_prev) {
+                const int64_t fc = rl64(lcost, 0), pc = rl64(lcost, 1);
+                if (pc < (field_valid ? fc : INT64_MAX)) { cur.pmx = prx; cur.pmy = pry; }
+            }
+            best.cost = (int64_t)best.sad;
+            key_x = best.mvx;
+            key_y = best.mvy;
+            if (memo && !fresh) {  // (gathered with the block's inputs)
+                const uint32_t k0 = in_word(kInMemo), r0 = in_word(kInMemo + 1);
+                if (in_word(kInMemo + 3) != 0 && (int16_t)(k0 & 0xFFFFu) == key_x && (int16_t)(k0 >> 16) == key_y) {
+                    best = Best{(int16_t)(r0 & 0xFFFFu), (int16_t)(r0 >> 16), in_word(kInMemo + 2),
+                                join64(in_word(kInMemo + 4), in_word(kInMemo + 5))};
+                    memo_hit = true;
+                }
+            }
+            if (!memo_hit) {
+                const int stype = g.search_type;
+                if (stype == 3) {
+                    P = Pattern{kPatUmh, best.mvx, best.mvy, radius, 0, pat_count(kPatUmh, radius, 0)};
+                    (void)eval_site(integral_constant<int, kPatUmh>{});
+                }
+                if (stype == 2 || stype == 3) {  // exact_hex2
+                    hex_r = radius;
+                    bmx = best.mvx;
+                    bmy = best.mvy;
+                    if (hex_r > 1) {
+                        P = Pattern{kPatHexExp, bmx, bmy, 1, 1, pat_count(kPatHexExp, 1, 1)};
+                        int w = eval_site(integral_constant<int, kPatHexExp>{});
+                        // a winner among the first 6 points moves the centre; otherwise the
+                        // closing expanding(1, 1) was this pass's second list
+                        if (w >= 0 && w < 6) {
+                            dir = w;
+                            bmx += kHex2[dir + 1][0];
+                            bmy += kHex2[dir + 1][1];
+                            hi = 1;
+                            while (hi < hex_r / 2 && in_bounds(cur, bmx, bmy)) {
+                                odir = kMod6m1[dir + 1];
+                                P = Pattern{kPatHexStep, bmx, bmy, 0, odir, 3};
+                                w = eval_site(integral_constant<int, kPatHexStep>{});
+                                if (w < 0) break;
+                                dir = odir + w - 1;
+                                bmx += kHex2[dir + 1][0];
+                                bmy += kHex2[dir + 1][1];
+                                hi++;
+                            }
+                            best.mvx = bmx;
+                            best.mvy = bmy;
+                            P = Pattern{kPatExpand, bmx, bmy, 1, 1, pat_count(kPatExpand, 1, 1)};
+                            (void)eval_site(integral_constant<int, kPatExpand>{});
+                        }
+                    } else {
+                        P = Pattern{kPatExpand, bmx, bmy, 1, 1, pat_count(kPatExpand, 1, 1)};
+                        (void)eval_site(integral_constant<int, kPatExpand>{});
+                    }
+                } else if (stype == 4) {
+                    P = Pattern{kPatExh, best.mvx, best.mvy, radius, 0, pat_count(kPatExh, radius, 0)};
+                    (void)eval_site(integral_constant<int, kPatExh>{});
+                }
+            }
+            if (memo && !memo_hit && !provisional && lane == 0)
+                *memo = Memo{(int16_t)key_x, (int16_t)key_y, (int16_t)best.mvx, (int16_t)best.mvy, best.sad, 1u,
+                             best.cost};
+            found = best.sad;
+        }
+    }
+    for (bool generic = !(kL0 && kFast); generic;) {
+        // ---------------- the evaluation site: kPerPass candidates per pass, one group each
+        const int win = eval_site(std::integral_constant<int, -1>{});
         // ---------------- act on the result; pick the next list (or finish)
         bool next = false;
         while (!next && step != kStEnd) {
