@@ -764,7 +764,9 @@ enum : int {
 // kL0: the finest level of a multi-level search with pnew >= -256, as compile-time
 // facts (no edge row/column, no half-size block, no bad-block pass, no adaptive radius,
 // no serial replay), so that the instantiation carries none of their state.
-template <bool kRefine, bool kFast = false, bool kL0 = false>
+// kLean: a coarser level whose chains all have pnew >= -256 (no serial replay).  With
+// kFast, both run the search's steps as straight-line code instead of the step machine.
+template <bool kRefine, bool kFast = false, bool kL0 = false, bool kLean = false>
 __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g, Blk& b, int idx,
                                              int bad_count, bool* bad, uint32_t* found_out, Memo* memo,
                                              SeedRec* rec, bool* unchanged, const RefineStart& rs,
@@ -922,7 +924,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
     Search cur;
     cur.pnew = kRefine ? rs.pnew : g.pnew;
     cur.chroma = !smallest;
-    cur.serial = !kL0 && cur.pnew < -256;
+    cur.serial = !kL0 && !kLean && cur.pnew < -256;
     cur.bnd[0] = -g.pel * px; cur.bnd[1] = -g.pel * py;
     cur.bnd[2] = g.pel * (g.lw - px - bw) + 1; cur.bnd[3] = g.pel * (g.lh - py - bh) + 1;
     cur.pmx = pred.dx; cur.pmy = pred.dy;
@@ -1135,17 +1137,52 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
         }
         return win;
     };
-    if constexpr (kL0 && kFast) {
-        // The finest level's steps as straight-line code (the step machine below with its
+    constexpr bool kLine = kFast && (kL0 || kLean);
+    if constexpr (kLine) {
+        // The search's steps as straight-line code (the step machine below with its
         // transitions fixed: predictor list, then UMH's lists and hex2's, hex2's alone or
-        // the exhaustive window, then the memo), each list evaluated by the evaluation
-        // site specialised for its kind.  No edge row or column, no bad-block pass and no
-        // serial replay at this level; the tile-plan instantiation only (the general cost
-        // path, specialised five times, would multiply the later sweeps' code).
+        // the exhaustive window, the memo, and at coarse levels the bad-block pass), each
+        // list evaluated by the evaluation site specialised for its kind.  The tile-plan
+        // instantiations only, without the serial replay (the general cost path,
+        // specialised five times, would multiply the later sweeps' code).
         using std::integral_constant;
+        // exact_hex2 around the best vector with radius r, closing with expanding(1, 1)
+        auto hex2 = [&](int r) {
+            bmx = best.mvx;
+            bmy = best.mvy;
+            if (r > 1) {
+                P = Pattern{kPatHexExp, bmx, bmy, 1, 1, pat_count(kPatHexExp, 1, 1)};
+                int w = eval_site(integral_constant<int, kPatHexExp>{});
+                // a winner among the first 6 points moves the centre; otherwise the
+                // closing expanding(1, 1) was this pass's second list
+                if (w >= 0 && w < 6) {
+                    dir = w;
+                    bmx += kHex2[dir + 1][0];
+                    bmy += kHex2[dir + 1][1];
+                    hi = 1;
+                    while (hi < r / 2 && in_bounds(cur, bmx, bmy)) {
+                        odir = kMod6m1[dir + 1];
+                        P = Pattern{kPatHexStep, bmx, bmy, 0, odir, 3};
+                        w = eval_site(integral_constant<int, kPatHexStep>{});
+                        if (w < 0) break;
+                        dir = odir + w - 1;
+                        bmx += kHex2[dir + 1][0];
+                        bmy += kHex2[dir + 1][1];
+                        hi++;
+                    }
+                    best.mvx = bmx;
+                    best.mvy = bmy;
+                    P = Pattern{kPatExpand, bmx, bmy, 1, 1, pat_count(kPatExpand, 1, 1)};
+                    (void)eval_site(integral_constant<int, kPatExpand>{});
+                }
+            } else {
+                P = Pattern{kPatExpand, bmx, bmy, 1, 1, pat_count(kPatExpand, 1, 1)};
+                (void)eval_site(integral_constant<int, kPatExpand>{});
+            }
+        };
         (void)eval_site(integral_constant<int, kPatPred>{});
-        cur.lambda = adapt_lambda(g.lambda_base, g.lsad, pred.score);
-        if (g.early) {  // distance 0: the predictor (or zero) is the answer
+        cur.lambda = adapt_lambda(smallest ? 0 : g.lambda_base, g.lsad, pred.score);
+        if (g.early) {  // level 0 with distance 0: the predictor (or zero) is the answer
             const int src = field_valid ? 0 : 3;
             best.mvx = rl(lx, src);
             best.mvy = rl(ly, src);
@@ -1169,44 +1206,15 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                 }
             }
             if (!memo_hit) {
-                const int stype = g.search_type;
+                // (the edge blocks that reach here are the smallest level's)
+                const int stype = (edge && level == 1) ? 0 : g.search_type;
                 if (stype == 3) {
                     P = Pattern{kPatUmh, best.mvx, best.mvy, radius, 0, pat_count(kPatUmh, radius, 0)};
                     (void)eval_site(integral_constant<int, kPatUmh>{});
                 }
-                if (stype == 2 || stype == 3) {  // exact_hex2
+                if (stype == 2 || stype == 3) {
                     hex_r = radius;
-                    bmx = best.mvx;
-                    bmy = best.mvy;
-                    if (hex_r > 1) {
-                        P = Pattern{kPatHexExp, bmx, bmy, 1, 1, pat_count(kPatHexExp, 1, 1)};
-                        int w = eval_site(integral_constant<int, kPatHexExp>{});
-                        // a winner among the first 6 points moves the centre; otherwise the
-                        // closing expanding(1, 1) was this pass's second list
-                        if (w >= 0 && w < 6) {
-                            dir = w;
-                            bmx += kHex2[dir + 1][0];
-                            bmy += kHex2[dir + 1][1];
-                            hi = 1;
-                            while (hi < hex_r / 2 && in_bounds(cur, bmx, bmy)) {
-                                odir = kMod6m1[dir + 1];
-                                P = Pattern{kPatHexStep, bmx, bmy, 0, odir, 3};
-                                w = eval_site(integral_constant<int, kPatHexStep>{});
-                                if (w < 0) break;
-                                dir = odir + w - 1;
-                                bmx += kHex2[dir + 1][0];
-                                bmy += kHex2[dir + 1][1];
-                                hi++;
-                            }
-                            best.mvx = bmx;
-                            best.mvy = bmy;
-                            P = Pattern{kPatExpand, bmx, bmy, 1, 1, pat_count(kPatExpand, 1, 1)};
-                            (void)eval_site(integral_constant<int, kPatExpand>{});
-                        }
-                    } else {
-                        P = Pattern{kPatExpand, bmx, bmy, 1, 1, pat_count(kPatExpand, 1, 1)};
-                        (void)eval_site(integral_constant<int, kPatExpand>{});
-                    }
+                    hex2(hex_r);
                 } else if (stype == 4) {
                     P = Pattern{kPatExh, best.mvx, best.mvy, radius, 0, pat_count(kPatExh, radius, 0)};
                     (void)eval_site(integral_constant<int, kPatExh>{});
@@ -1216,9 +1224,32 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                 *memo = Memo{(int16_t)key_x, (int16_t)key_y, (int16_t)best.mvx, (int16_t)best.mvy, best.sad, 1u,
                              best.cost};
             found = best.sad;
+            if constexpr (!kL0) {
+                // bad-block expansion at coarse levels (:607-672), around the zero vector
+                const int bad_limit = sat_add_i32(g.bad_sad, sat_mul_i32(g.bad_sad, bad_count) / 16);
+                if (level > 0 && idx > 1 && g.bad_sad > 0 && (int)best.sad > bad_limit) {
+                    is_bad = true;
+                    cur.pmx = pred.dx;  // the block's own predictor
+                    cur.pmy = pred.dy;
+                    if (g.bad_range < 0) {  // rings of growing radius until one quarters the sad
+                        for (ring = g.pel; ring < -g.bad_range * g.pel; ring += g.pel) {
+                            P = Pattern{kPatExpand, 0, 0, ring, g.pel, pat_count(kPatExpand, ring, g.pel)};
+                            (void)eval_site(integral_constant<int, kPatExpand>{});
+                            if (best.sad < found / 4) break;
+                        }
+                    } else if (g.bad_range > 0) {
+                        hex_r = g.bad_range * g.pel;
+                        P = Pattern{kPatUmh, 0, 0, hex_r, 0, pat_count(kPatUmh, hex_r, 0)};
+                        (void)eval_site(integral_constant<int, kPatUmh>{});
+                        hex2(hex_r);
+                    }
+                    P = Pattern{kPatExpand, best.mvx, best.mvy, 1, 1, pat_count(kPatExpand, 1, 1)};
+                    (void)eval_site(integral_constant<int, kPatExpand>{});
+                }
+            }
         }
     }
-    for (bool generic = !(kL0 && kFast); generic;) {
+    for (bool generic = !kLine; generic;) {
         // ---------------- the evaluation site: kPerPass candidates per pass, one group each
         const int win = eval_site(std::integral_constant<int, -1>{});
         // ---------------- act on the result; pick the next list (or finish)
@@ -1621,7 +1652,7 @@ __global__ void __launch_bounds__(kScanThreads) sweep_scan_kernel(const Chain* _
 // result stored as any other) goes to the list of sweep s that deferred_kernel
 // completes before sweep s + 1.
 // Returns whether the block's stored vector changed; *me_out = its visit position.
-template <bool kFast, bool kL0 = false>
+template <bool kFast, bool kL0 = false, bool kLean = false>
 __device__ __forceinline__ bool sweep_item(const Chain* __restrict__ chains, int ci, int idx, const SweepState& st,
                                            int s, int* me_out = nullptr, int clear_parity = -1) {
     // the descriptor read in place through the constant address space: scalar loads the
@@ -1641,7 +1672,7 @@ __device__ __forceinline__ bool sweep_item(const Chain* __restrict__ chains, int
     io.found = st.found + ch.base + idx;
     io.badf = st.badf + ch.base + idx;
     io.fresh = s == 0;  // sweep_init cleared every memo and seed record of the level
-    const DVec o = search_block<false, kFast, kL0>(ch, g, b, idx, 0, &bad, &found, st.memo + ch.base + idx,
+    const DVec o = search_block<false, kFast, kL0, kLean>(ch, g, b, idx, 0, &bad, &found, st.memo + ch.base + idx,
                                               st.seeds + ch.base + idx, &unchanged, RefineStart{}, &defer, &io);
     if (kFast && defer != kDeferNone) {
         if (lane_id() == 0) st.dl[atomicAdd(&st.dcount[kDcDeferred], 1u)] = ((unsigned long long)ci << 32) | (unsigned)idx;
@@ -1706,22 +1737,23 @@ __device__ __forceinline__ void level0_order(int pos, int gw, int gh, int& x, in
 // neighbours have finished already seeds from their searched vectors, and
 // validate_kernel finds its seed record up to date.
 // kFast: the tile-plan instantiation (three waves per SIMD), which passes the blocks it
-// cannot finish to deferred_kernel.  kRun: positions per item, searched in order by one
+// cannot finish to deferred_kernel.  run: positions per item, searched in order by one
 // wave.  The coarse levels' order lists each sorted parent's children together, so with
-// runs of 8 (two parents) a block's earlier siblings are mostly stored before it reads
-// them; the finest level's meander is served best by single positions.
-template <bool kFast, int kRun, bool kL0 = false>
+// runs of 8 or more a block's earlier siblings are mostly stored before it reads them;
+// at the finest level consecutive meander positions keep the reference rows of a
+// block's window in the caches for the next.
+template <bool kFast, bool kL0 = false, bool kLean = false>
 __global__ void __launch_bounds__(kWave, kFast ? 3 : 2) sweep0_kernel(const Chain* __restrict__ chains,
                                                                      SweepState st, int nslots, int nseg,
-                                                                     long long total) {
+                                                                     int run, long long total) {
     for (long long u = blockIdx.x; u < total; u += gridDim.x) {
         const int v = (int)(u % nslots), k0 = (int)(u / nslots);
         const int ci = v / nseg, seg = v - ci * nseg;
         const LevelGeom* g = &chains[ci].geom;
         const int gw = g->gw, gh = g->gh;
         const int n = gw * gh, len = (n + nseg - 1) / nseg;
-        for (int j = 0; j < kRun; j++) {
-            const int k = k0 * kRun + j;
+        for (int j = 0; j < run; j++) {
+            const int k = k0 * run + j;
             const int pos = seg * len + k;
             if (k >= len || pos >= n) break;
             int bx, by;
@@ -1733,9 +1765,9 @@ __global__ void __launch_bounds__(kWave, kFast ? 3 : 2) sweep0_kernel(const Chai
                 by = pk >> 16;
             }
             if (bx >= gw || by >= gh) continue;
-            sweep_item<kFast, kL0>(chains, ci, by * g->gw + bx, st, 0);
+            sweep_item<kFast, kL0, kLean>(chains, ci, by * g->gw + bx, st, 0);
             // the next position's seed reads see this block's result
-            if (kRun > 1) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            if (run > 1) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         }
     }
 }
@@ -2258,7 +2290,7 @@ hipError_t launch_sweep_init(const Chain* d_chains, int nchains, int max_n, cons
 }
 
 hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s,
-                         int level, bool fast, bool l0, double slot_scale, int coarse_slots, int fine_run,
+                         int level, bool fast, bool l0, bool lean, double slot_scale, int coarse_slots, int fine_run,
                          int coarse_run) {
     // segments per chain: a multiple of q so that the slot count is a multiple of the 8
     // XCDs, and no segment much shorter than seg_min blocks.  Fewer slots put more items
@@ -2297,38 +2329,18 @@ hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const Sw
     const int nslots = nchains * nseg;
     const long long total = (long long)nslots * (((max_n + nseg - 1) / nseg + run - 1) / run);
     const unsigned grid = (unsigned)std::min<long long>(total, 1 << 22);
-    if (fast && l0 && run == 1)
-        hipLaunchKernelGGL((sweep0_kernel<true, 1, true>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg,
+    if (fast && l0)
+        hipLaunchKernelGGL((sweep0_kernel<true, true>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, run,
                            total);
-    else if (fast && l0 && run == 2)
-        hipLaunchKernelGGL((sweep0_kernel<true, 2, true>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg,
-                           total);
-    else if (fast && l0 && run == 4)
-        hipLaunchKernelGGL((sweep0_kernel<true, 4, true>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg,
-                           total);
-    else if (fast && l0 && run == 16)
-        hipLaunchKernelGGL((sweep0_kernel<true, 16, true>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg,
-                           total);
-    else if (fast && l0 && run == 32)
-        hipLaunchKernelGGL((sweep0_kernel<true, 32, true>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg,
-                           total);
-    else if (fast && l0)
-        hipLaunchKernelGGL((sweep0_kernel<true, 8, true>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg,
-                           total);
-    else if (fast && run == 1)
-        hipLaunchKernelGGL((sweep0_kernel<true, 1>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
-    else if (fast && run == 4)
-        hipLaunchKernelGGL((sweep0_kernel<true, 4>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
-    else if (fast && run == 16)
-        hipLaunchKernelGGL((sweep0_kernel<true, 16>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
-    else if (fast && run == 32)
-        hipLaunchKernelGGL((sweep0_kernel<true, 32>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
+    else if (fast && lean)
+        hipLaunchKernelGGL((sweep0_kernel<true, false, true>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots,
+                           nseg, run, total);
     else if (fast)
-        hipLaunchKernelGGL((sweep0_kernel<true, 8>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
-    else if (run == 1)
-        hipLaunchKernelGGL((sweep0_kernel<false, 1>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
+        hipLaunchKernelGGL((sweep0_kernel<true>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, run,
+                           total);
     else
-        hipLaunchKernelGGL((sweep0_kernel<false, 8>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, total);
+        hipLaunchKernelGGL((sweep0_kernel<false>), dim3(grid), dim3(kWave), 0, s, d_chains, st, nslots, nseg, run,
+                           total);
     return hipGetLastError();
 }
 

@@ -13,7 +13,7 @@ namespace svpf {
 hipError_t launch_search_level(const Chain* d_chains, int nchains, hipStream_t s);
 hipError_t launch_sweep_init(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s);
 hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s,
-                         int level, bool fast, bool l0, double slot_scale, int coarse_slots, int fine_run,
+                         int level, bool fast, bool l0, bool lean, double slot_scale, int coarse_slots, int fine_run,
                          int coarse_run);
 hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s, bool l0);
 hipError_t launch_sweep_stats(const SweepState& st, int sweeps, long long total_blocks, unsigned long long* out4,
@@ -222,9 +222,10 @@ int sweep_graph(svpf_ctx* ctx, const Chain* dch, int nch, int grid, const SweepS
 // chunk, and sweeps launched past convergence find an empty queue and return at once.
 // fast: every chain's searched block is a 4x4 or 8x8 square, and sweep 0 (every block
 // of the level) runs the tile-plan kernel, then the general one over the blocks it
-// passed on.
+// passed on.  l0: the finest level's instantiation; lean: every chain's pnew >= -256,
+// so the tile-plan kernel runs its steps as straight-line code.
 int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total_blocks, const SweepState& st,
-               bool scan, int level, bool fast, bool l0, bool no_alloc) {
+               bool scan, int level, bool fast, bool l0, bool lean, bool no_alloc) {
     const size_t words = 4;
     uint32_t local_count = 0;  // no_alloc: read back through pageable memory (a synchronous copy)
     uint32_t* h_count = no_alloc ? &local_count : nullptr;
@@ -237,7 +238,7 @@ int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total
     }
     if (!no_alloc) h_count = ctx->h_count;
     const int grid = (int)std::min<size_t>(std::max<size_t>(total_blocks, 1), (size_t)ctx->sweep_grid);
-    SVPF_HIP(launch_sweep0(dch, nch, max_n, st, ctx->stream, level, fast, l0, ctx->slot_scale, ctx->coarse_slots,
+    SVPF_HIP(launch_sweep0(dch, nch, max_n, st, ctx->stream, level, fast, l0, lean, ctx->slot_scale, ctx->coarse_slots,
                            ctx->fine_run, ctx->coarse_run));
     if (fast) SVPF_HIP(launch_deferred(dch, grid, st, 0, ctx->stream));
     if (scan) SVPF_HIP(launch_sweep_scan(dch, nch, st, 0, ctx->stream));
@@ -538,8 +539,10 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
     bool seq_step[16] = {}, bad_step[16] = {};  // sequential fallback / bad-block count rebuilds
     bool plan_step[16];                         // every searched block a 4x4 or 8x8 square
     bool l0_step[16];                           // the finest level of multi-level searches, pnew >= -256
+    bool lean_step[16];                         // every chain's pnew >= -256 (no serial replay)
     for (bool& f : plan_step) f = true;
     for (int l = 0; l < 16; l++) l0_step[l] = l == 0;
+    for (bool& f : lean_step) f = true;
     Transition* inits = (Transition*)&host[init_off];
     int ninit = 0, max_init_n = 0;
     for (int step = max_levels - 1; step >= 0; step--) {
@@ -643,6 +646,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
                 const int sw = g.smallest ? g.bw >> 1 : g.bw, sh = g.smallest ? g.bh >> 1 : g.bh;
                 if (sw != sh || (sw != 4 && sw != 8)) plan_step[l] = false;
                 if (g.level != 0 || g.smallest || g.pnew < -256) l0_step[l] = false;
+                if (g.pnew < -256) lean_step[l] = false;
             }
         }
     }
@@ -789,7 +793,7 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
             SVPF_HIP(launch_sweep_init(dch, nch_step[l], level_max_n[l], st, ctx->stream));
             int rc2 = seq_step[l] ? SVPF_OK
                                   : run_sweeps(ctx, dch, nch_step[l], level_max_n[l], level_blocks[l], st, bad_step[l],
-                                               l, plan_step[l], l0_step[l], ext != nullptr);
+                                               l, plan_step[l], l0_step[l], lean_step[l], ext != nullptr);
             if (rc2 == kNotConverged || seq_step[l])
                 SVPF_HIP(launch_search_level(dch, nch_step[l], ctx->stream));
             else if (rc2)

@@ -89,6 +89,11 @@ def test_analyse_cfg3_settings_small(gpu_ctx, oracle):
     "{block:{w:8,h:8,overlap:0}, main:{search:{type:4,distance:0}}}",
     "{block:{w:8,h:8,overlap:3}, main:{penalty:{pnew:-300, pzero:10}, search:{type:2,distance:6}}}",
     "{block:{w:16,h:16,overlap:2}, main:{search:{type:3,distance:12,coarse:{bad:{range:6, sad:200}}}}}",
+    # tile-plan coarse levels: bad-block UMH + hex2 (range > 0), rings (range < 0) with a low
+    # threshold, and two levels (the smallest level is level 1: its edge blocks skip refinement)
+    "{block:{w:8,h:8,overlap:2}, main:{search:{type:3,distance:12,coarse:{bad:{range:6, sad:150}}}}}",
+    "{block:{w:4,h:4,overlap:0}, main:{search:{type:2,distance:6,coarse:{bad:{range:-3, sad:40}}}}}",
+    "{block:{w:8,h:8,overlap:2}, main:{levels:2, search:{type:3,distance:8,coarse:{bad:{sad:200}}}}}",
     "{block:{w:8,h:8,overlap:3}, main:{search:{type:2,distance:8,coarse:{distance:-6, satd:false}}}}",
     "{block:{w:4,h:4,overlap:0}, main:{search:{type:2,distance:4,satd:true}}}",
     "{block:{w:32,h:32,overlap:3}, main:{search:{type:3,distance:8,satd:true}}}",
