@@ -49,6 +49,14 @@ using namespace cost;
 // Lane of the calling thread within its wavefront (kernels here run one or more whole
 // wavefronts per workgroup).
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
+// The lane index as a value the compiler cannot carry across block evaluations: the lane
+// masks an evaluation derives from it are computed there, not kept (and spilled) across
+// a kernel's loop.
+__device__ __forceinline__ int lane_here() {
+    int l = lane_id();
+    asm volatile("" : "+v"(l));
+    return l;
+}
 
 // One of three values by a per-lane index, selected as values: a conditional over
 // struct members is an lvalue, which the compiler turns into a selected address and a
@@ -292,6 +300,24 @@ typedef __attribute__((address_space(1))) const uint8_t g_u8;
 // Chain descriptors are written by the host before a launch and only read by kernels.
 typedef __attribute__((address_space(4))) const struct Chain const_chain;
 typedef __attribute__((address_space(1))) const uint32_t __attribute__((aligned(1))) g_u32u;
+typedef __attribute__((address_space(4))) const struct SweepState const_sweep;
+typedef __attribute__((address_space(4))) const char const_byte;
+// The leading arguments of the kernels that read the sweep state this way (the chain
+// descriptors, then the state): its place in the kernel-argument segment.
+struct SweepArgsHead {
+    const Chain* chains;
+    SweepState st;
+};
+// The kernel's sweep-state argument as seen by one block evaluation: read in place in
+// the kernel-argument segment through a pointer the compiler cannot follow across
+// evaluations, so that the fields load (scalar loads) where an evaluation uses them
+// instead of living across the kernel's loop.  (The address of the by-value parameter
+// itself would be a private copy's.)
+__device__ __forceinline__ const SweepState& state_here() {
+    const_sweep* p = (const_sweep*)((const_byte*)__builtin_amdgcn_kernarg_segment_ptr() + offsetof(SweepArgsHead, st));
+    asm volatile("" : "+s"(p));
+    return *(const SweepState*)p;
+}
 __device__ __forceinline__ uint32_t gld32(const g_u8* p) { return *reinterpret_cast<const g_u32u*>(p); }
 
 // Each lane of a group (part = lane % 4) owns the luma 4x4 tile `part` of an 8x8 block
@@ -771,7 +797,7 @@ __device__ __forceinline__ DVec search_block(const Chain& ch, const LevelGeom& g
                                              int bad_count, bool* bad, uint32_t* found_out, Memo* memo,
                                              SeedRec* rec, bool* unchanged, const RefineStart& rs,
                                              int* defer = nullptr, BlockIO* io = nullptr) {
-    const int lane = lane_id();
+    const int lane = lane_here();
     const int part = lane & (kGroup - 1);
     const int gw = g.gw, gh = g.gh, level = kL0 ? 0 : g.level;
     const bool smallest = !kRefine && !kL0 && g.smallest != 0;
@@ -1765,7 +1791,7 @@ __global__ void __launch_bounds__(kWave, kFast ? 3 : 2) sweep0_kernel(const Chai
                 by = pk >> 16;
             }
             if (bx >= gw || by >= gh) continue;
-            sweep_item<kFast, kL0, kLean>(chains, ci, by * g->gw + bx, st, 0);
+            sweep_item<kFast, kL0, kLean>(chains, ci, by * g->gw + bx, state_here(), 0);
             // the next position's seed reads see this block's result
             if (run > 1) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         }
@@ -1856,7 +1882,7 @@ __global__ void __launch_bounds__(kWave, 2) sweep_kernel(const Chain* __restrict
         const int ci = __builtin_amdgcn_readfirstlane((int)(item >> 32));
         int idx = __builtin_amdgcn_readfirstlane((int)(item & 0xFFFFFFFFu));
         if (chase == 0) {  // every queued block once (its flag), no claim needed
-            sweep_item<false, kL0>(chains, ci, idx, st, s, nullptr, p);
+            sweep_item<false, kL0>(chains, ci, idx, state_here(), s, nullptr, p);
             continue;
         }
         const uint32_t base = (uint32_t)uniform_int((int)chains[ci].base);
@@ -1867,7 +1893,7 @@ __global__ void __launch_bounds__(kWave, 2) sweep_kernel(const Chain* __restrict
         }
         for (int k = 0;; k++) {
             int me = 0;
-            if (!sweep_item<false, kL0>(chains, ci, idx, st, s, &me) || k >= chase) break;
+            if (!sweep_item<false, kL0>(chains, ci, idx, state_here(), s, &me) || k >= chase) break;
             const int next = order_successor(chains, ci, idx, me);
             if (next < 0 || !claim_block(st, base, next, s)) break;
             // this wave's store of idx before the successor reads it (the same wave: a
@@ -1893,7 +1919,7 @@ __global__ void __launch_bounds__(kWave, 2) deferred_kernel(const Chain* __restr
         const unsigned long long item = st.dl[i];
         const int ci = __builtin_amdgcn_readfirstlane((int)(item >> 32));
         const int idx = __builtin_amdgcn_readfirstlane((int)(item & 0xFFFFFFFFu));
-        sweep_item<false>(chains, ci, idx, st, s);
+        sweep_item<false>(chains, ci, idx, state_here(), s);
     }
 }
 
