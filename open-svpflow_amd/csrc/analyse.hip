@@ -1547,6 +1547,9 @@ __global__ void sweep_init_kernel(const Chain* __restrict__ chains, SweepState s
         st.found[ch.base + i] = 0;
         st.badf[ch.base + i] = 0;
         st.cpre[ch.base + i] = 1 << 30;  // sweep 0 runs no bad-block pass: the limit saturates
+        st.fseq[ch.base + i] = 0;         // (position i: the same three words)
+        st.bseq[ch.base + i] = 0;
+        st.cseq[ch.base + i] = 1 << 30;
         st.memo[ch.base + i].valid = 0;
         st.seeds[ch.base + i].valid = 0;
         st.stamp[ch.base + i] = 0;
@@ -1598,10 +1601,19 @@ __device__ __forceinline__ int block_excl_scan(int v, int* wsum) {
 
 // The running bad-block count after positions [lo, hi) of the visit order, from input
 // c.  fseq holds each position's pre-expansion cost, 0 for blocks that never count
-// (the first two raster blocks): with bad_sad > 0 the limit is at least bad_sad.
+// (the first two raster blocks): with bad_sad > 0 the limit is at least bad_sad.  The
+// costs are loaded kScanBatch at a time (independent loads, one wait per batch).
+constexpr int kScanBatch = 8;
+__device__ __forceinline__ int bad_limit(int bad_sad, int c) { return sat_add_i32(bad_sad, sat_mul_i32(bad_sad, c) / 16); }
 __device__ __forceinline__ int scan_walk(const uint32_t* fseq, int lo, int hi, int bad_sad, int c) {
-    for (int k = lo; k < hi; k++)
-        if ((int)fseq[k] > sat_add_i32(bad_sad, sat_mul_i32(bad_sad, c) / 16)) c++;
+    for (int k0 = lo; k0 < hi; k0 += kScanBatch) {
+        uint32_t f[kScanBatch];
+#pragma unroll
+        for (int j = 0; j < kScanBatch; j++) f[j] = k0 + j < hi ? fseq[k0 + j] : 0u;
+#pragma unroll
+        for (int j = 0; j < kScanBatch; j++)
+            if ((int)f[j] > bad_limit(bad_sad, c)) c++;
+    }
     return c;
 }
 
@@ -1616,7 +1628,9 @@ __device__ __forceinline__ int scan_walk(const uint32_t* fseq, int lo, int hi, i
 //     in[t] = in[0] + sum over u < t of (walk(u, in[u]) - in[u])
 // from the previous rebuild's counts: chunk 0's input is exact, so chunk t is exact
 // after t iterations and a repeat with no change is the sequential result (usually
-// after one or two iterations, since a sweep changes few costs).
+// after one or two iterations, since a sweep changes few costs).  Every read is of the
+// visit-order copies; a block's raster index is looked up only where its count or its
+// bad flag changes.
 __device__ void scan_chain(const Chain* __restrict__ chains, int ci, const SweepState& st, int s,
                            int* sh) {
     const Chain ch = uniform_load(&chains[ci]);
@@ -1629,21 +1643,14 @@ __device__ void scan_chain(const Chain* __restrict__ chains, int ci, const Sweep
     const int n = g.gw * g.gh;
     const int m = (n - (int)start + T - 1) / T;
     const int lo = min((int)start + tid * m, n), hi = min(lo + m, n);
-    uint32_t* fs = st.fseq + ch.base;
-    int c0 = 0;
-    if (start > 0) {
-        const int p0 = ch.order[start];
-        c0 = st.cpre[ch.base + (p0 >> 16) * g.gw + (p0 & 0xFFFF)];
-    }
+    const uint32_t* fs = st.fseq + ch.base;
+    const uint32_t* bs = st.bseq + ch.base;
+    int32_t* cs = st.cseq + ch.base;
+    const int c0 = start > 0 ? cs[start] : 0;
     int guess = c0;
-    for (int k = lo; k < hi; k++) {
-        const int pk = ch.order[k];
-        const int idx = (pk >> 16) * g.gw + (pk & 0xFFFF);
-        if (k == lo && tid > 0) {
-            const int old = st.cpre[ch.base + idx];
-            guess = old < (1 << 30) ? old : c0;  // before the first rebuild of the level: no guess
-        }
-        fs[k] = idx > 1 ? st.found[ch.base + idx] : 0u;
+    if (tid > 0 && lo < hi) {
+        const int old = cs[lo];
+        guess = old < (1 << 30) ? old : c0;  // before the first rebuild of the level: no guess
     }
     int* wsum = sh + T;
     int in = guess;
@@ -1655,13 +1662,34 @@ __device__ void scan_chain(const Chain* __restrict__ chains, int ci, const Sweep
         if (!__syncthreads_or(changed)) break;
     }
     int c = in;
-    for (int k = lo; k < hi; k++) {
-        const int pk = ch.order[k];
-        const int idx = (pk >> 16) * g.gw + (pk & 0xFFFF);
-        const int bj = (int)fs[k] > sat_add_i32(g.bad_sad, sat_mul_i32(g.bad_sad, c) / 16) ? 1 : 0;
-        st.cpre[ch.base + idx] = c;
-        queue_wave(st, ci, ch, idx, (uint32_t)bj != st.badf[ch.base + idx], s);
-        c += bj;
+    for (int k0 = lo; k0 < hi; k0 += kScanBatch) {
+        uint32_t f[kScanBatch], b[kScanBatch];
+        int cc[kScanBatch];
+#pragma unroll
+        for (int j = 0; j < kScanBatch; j++) {
+            const bool v = k0 + j < hi;
+            f[j] = v ? fs[k0 + j] : 0u;
+            b[j] = v ? bs[k0 + j] : 0u;
+            cc[j] = v ? cs[k0 + j] : c;
+        }
+#pragma unroll
+        for (int j = 0; j < kScanBatch; j++) {
+            const int k = k0 + j;
+            const bool v = k < hi;
+            const int bj = (int)f[j] > bad_limit(g.bad_sad, c) ? 1 : 0;
+            const bool cchg = v && cc[j] != c, want = v && (uint32_t)bj != b[j];
+            int idx = 0;
+            if (cchg || want) {
+                const int pk = ch.order[k];
+                idx = (pk >> 16) * g.gw + (pk & 0xFFFF);
+            }
+            if (cchg) {
+                st.cpre[ch.base + idx] = c;
+                cs[k] = c;
+            }
+            queue_wave(st, ci, ch, idx, want, s);
+            if (v) c += bj;
+        }
     }
     if (tid == 0) st.minpos[ci] = 0xFFFFFFFFu;
 }
@@ -1720,7 +1748,11 @@ __device__ __forceinline__ bool sweep_item(const Chain* __restrict__ chains, int
         if (found != oldf || (uint32_t)bad != oldb) {
             st.found[ch.base + idx] = found;
             st.badf[ch.base + idx] = bad ? 1u : 0u;
-            if (g.level > 0 && g.bad_sad > 0) atomicMin(&st.minpos[ci], (uint32_t)me);
+            if (!kL0 && g.level > 0 && g.bad_sad > 0) {  // the count rebuild's copies, by visit position
+                st.fseq[ch.base + me] = idx > 1 ? found : 0u;
+                st.bseq[ch.base + me] = bad ? 1u : 0u;
+                atomicMin(&st.minpos[ci], (uint32_t)me);
+            }
         }
     }
     // sweep 0 queues nothing: validate_kernel compares every block's seed record with

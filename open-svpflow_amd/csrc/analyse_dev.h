@@ -90,8 +90,9 @@ static_assert(sizeof(SeedRec) == 48 && offsetof(SeedRec, valid) == 40, "SeedRec 
 // Coarse levels with the bad-block pass also keep, per block, the pre-expansion cost
 // (`found`), whether the expansion ran (`badf`) and the running count of bad blocks
 // visited before it (`cpre`, rebuilt after every sweep from the first visit position
-// whose found/bad changed, `minpos` per chain, with `fseq` as the rebuild's copy of
-// the costs in visit order).
+// whose found/bad changed, `minpos` per chain).  The rebuild reads copies of the three
+// in visit order (`fseq`, `bseq`, `cseq`), which the evaluations and the rebuild keep
+// equal to the per-block words, so its reads are contiguous and need no order lookup.
 constexpr int kCntQueued = 3, kCntSkipped = 4, kCntWords = 8;
 constexpr int kDcDeferred = 0, kDcChased = 1;
 
@@ -112,7 +113,9 @@ struct SweepState {
     SeedRec* seeds;             // per block
     unsigned long long* dl;     // blocks the tile-plan kernels passed on in the current sweep
     uint32_t* dcount;           // dcount[kDcDeferred]: their number; dcount[kDcChased]: chased evaluations
-    uint32_t* fseq;             // per block: pre-expansion costs in visit order (count rebuild)
+    uint32_t* fseq;             // per visit position: found, 0 for raster blocks 0 and 1 (never counted)
+    uint32_t* bseq;             // per visit position: badf
+    int32_t* cseq;              // per visit position: cpre
     uint32_t* stamp;            // per block: the last sweep (>= 1) whose evaluation claimed it
     int max_sweeps;
     int chase;                  // visit-order successors a later sweep evaluates right after a change

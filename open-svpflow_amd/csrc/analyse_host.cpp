@@ -405,7 +405,8 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
     }
     // sweep state per level: flags x2 (one entry per block of every chain), worklists x2, counters
     size_t level_blocks[16] = {}, flags_off[16][2], wl_off[16][2], counts_off[16], found_off[16], badf_off[16],
-        cpre_off[16], minpos_off[16], memo_off[16], fseq_off[16], seed_off[16], dl_off[16], stamp_off[16];
+        cpre_off[16], minpos_off[16], memo_off[16], fseq_off[16], bseq_off[16], cseq_off[16], seed_off[16], dl_off[16],
+        stamp_off[16];
     int level_max_n[16] = {};
     for (int l = 0; l < max_levels; l++) {
         size_t tot = 0;
@@ -430,6 +431,8 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
         minpos_off[l] = work.take(4 * (size_t)nchains + 4);
         memo_off[l] = work.take(sizeof(Memo) * (tot + 1));
         fseq_off[l] = work.take(4 * tot + 4);
+        bseq_off[l] = work.take(4 * tot + 4);
+        cseq_off[l] = work.take(4 * tot + 4);
         seed_off[l] = work.take(sizeof(SeedRec) * (tot + 1));
         stamp_off[l] = work.take(4 * tot + 4);
     }
@@ -786,6 +789,8 @@ int analyse_run(svpf_ctx* ctx, int n, const svpf_analyse_params* const* params, 
             st.minpos = (uint32_t*)(W + minpos_off[l]);
             st.memo = (Memo*)(W + memo_off[l]);
             st.fseq = (uint32_t*)(W + fseq_off[l]);
+            st.bseq = (uint32_t*)(W + bseq_off[l]);
+            st.cseq = (int32_t*)(W + cseq_off[l]);
             st.seeds = (SeedRec*)(W + seed_off[l]);
             st.stamp = (uint32_t*)(W + stamp_off[l]);
             st.chase = ctx->chase;
