@@ -195,14 +195,22 @@ def cpu_baseline(frames, pairs: int):
 
 
 def cpu_model() -> str:
+    """The host CPU as /proc/cpuinfo names it, with its family/model numbers (a virtual
+    machine may report only a generic model name)."""
+    info = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if not line.strip():
+                    break  # the first processor's entry
+                k, _, v = line.partition(":")
+                info[k.strip()] = v.strip()
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    name = info.get("model name", "unknown")
+    if "cpu family" in info and "model" in info:
+        name += f" (family {info['cpu family']}, model {info['model']})"
+    return name
 
 
 # ------------------------------------------------------------------------- config 4
