@@ -1901,15 +1901,18 @@ __device__ __forceinline__ int order_successor(const Chain* __restrict__ chains,
 // sweep would cost more than three waves per SIMD gain.)
 template <bool kL0>
 __global__ void __launch_bounds__(kWave, 2) sweep_kernel(const Chain* __restrict__ chains, SweepState st, int s) {
-    const uint32_t n = (uint32_t)uniform_int((int)__atomic_load_n(&st.counts[s % 3], __ATOMIC_RELAXED));
     const int p = s & 1;
+    // this workgroup's first item, loaded with the list length rather than after it (the
+    // grid is at most one workgroup per block of the level, so the slot exists)
+    const unsigned long long first = st.wl[p][blockIdx.x];
+    const uint32_t n = (uint32_t)uniform_int((int)__atomic_load_n(&st.counts[s % 3], __ATOMIC_RELAXED));
     const int chase = st.chase;
     if (blockIdx.x == 0 && lane_id() == 0) {
         st.counts[(s + 2) % 3] = 0;  // read by sweep s - 1; sweep s + 1 queues into it
         atomicAdd(&st.counts[kCntQueued], n);
     }
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
-        const unsigned long long item = st.wl[p][i];
+        const unsigned long long item = i == blockIdx.x ? first : st.wl[p][i];
         // uniform across the wave: keeps the chain's descriptors in scalar registers
         const int ci = __builtin_amdgcn_readfirstlane((int)(item >> 32));
         int idx = __builtin_amdgcn_readfirstlane((int)(item & 0xFFFFFFFFu));
