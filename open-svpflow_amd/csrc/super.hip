@@ -17,6 +17,12 @@ namespace {
 constexpr int kThreads = 256;
 
 __device__ __forceinline__ uint8_t avg8(uint32_t a, uint32_t b) { return (uint8_t)((a + b + 1) >> 1); }
+// avg8 of each of the four byte lanes of a and b: a + b = 2(a & b) + (a ^ b), so the
+// rounded-up half is (a | b) - ((a ^ b) >> 1) per byte (the shifted xor masked to 7 bits).
+__device__ __forceinline__ uint32_t avg8x4(uint32_t a, uint32_t b) { return (a | b) - (((a ^ b) >> 1) & 0x7F7F7F7Fu); }
+
+typedef __attribute__((address_space(1))) const uint8_t g_u8;
+typedef __attribute__((address_space(1))) const uint32_t __attribute__((aligned(1))) g_u32u;
 
 struct L0Args {
     SuperGeom g;
@@ -71,10 +77,29 @@ __global__ void __launch_bounds__(kThreads) super_level0_kernel(L0Args a) {
 
     extern __shared__ uint8_t lds[];
     const int pitch = ds + 4;
-    for (int k = 0; k < 6; k++) {  // rows y-2..y+3 (clamped), edge-padded to the stride
-        int r = min(max(y - 2 + k, 0), h - 1);
+    // rows y-2..y+3 (clamped), edge-padded to the stride: the words inside the row with
+    // one (unaligned) 4-byte load per row, six in flight, then the padded tail by bytes
+    const int wfull = w & ~3;
+    for (int c = threadIdx.x * 4; c < wfull; c += kThreads * 4) {
+        uint32_t v[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            const int r = min(max(y - 2 + k, 0), h - 1);
+            v[k] = *(const g_u32u*)((const g_u8*)sp + ((size_t)r * ss + c));
+        }
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            uint8_t* L = lds + k * pitch + c;
+            L[0] = (uint8_t)v[k];
+            L[1] = (uint8_t)(v[k] >> 8);
+            L[2] = (uint8_t)(v[k] >> 16);
+            L[3] = (uint8_t)(v[k] >> 24);
+        }
+    }
+    for (int k = 0; k < 6; k++) {
+        const int r = min(max(y - 2 + k, 0), h - 1);
         const uint8_t* s = sp + (size_t)r * ss;
-        for (int c = threadIdx.x; c < pitch; c += kThreads) lds[k * pitch + c] = s[min(c, w - 1)];
+        for (int c = wfull + threadIdx.x; c < pitch; c += kThreads) lds[k * pitch + c] = s[min(c, w - 1)];
     }
     __syncthreads();
     const Rows R{(lds_u8*)lds, sp, ss, pitch, y, h, w, ds};
@@ -85,6 +110,27 @@ __global__ void __launch_bounds__(kThreads) super_level0_kernel(L0Args a) {
     const bool bilinear = fill && g.scale_up == 0;
     const bool bicubic = fill && g.scale_up == 2;
 
+    // Bilinear pel 2 away from the edges (rows 1 .. h-2, pixels whose right neighbour
+    // lies inside the row): the four subplanes four pixels at a time from words of rows y
+    // and y+1 -- full F(y, x), x-half avg(F(y, x), F(y, x+1)), y-half avg(F(y, x),
+    // F(y+1, x)), diagonal avg(F(y, x), F(y+1, x+1)) (super_build.rs:183-223 off its
+    // first-row and past-the-end rules).  The rest of each row takes the loop below.
+    int done = 0;  // columns [0, done) written here
+    const bool aligned4 = ((((uintptr_t)dp) | (uintptr_t)ds | (uintptr_t)pitch) & 3) == 0;
+    if (bilinear && pel == 2 && aligned4 && y >= 1 && y + 1 < h) {
+        done = (w - 1) & ~3;  // x + 4 < w for every x of a word below `done`
+        const int r0 = g.level_row[plane][0];
+        const uint32_t* L0 = (const uint32_t*)(lds + 2 * pitch);  // row y (LDS row k = 2)
+        const uint32_t* L1 = (const uint32_t*)(lds + 3 * pitch);  // row y + 1
+        for (int c = threadIdx.x * 4; c < done; c += kThreads * 4) {
+            const uint32_t a = L0[c >> 2], an = L0[(c >> 2) + 1], b = L1[c >> 2], bn = L1[(c >> 2) + 1];
+            const uint32_t a1 = __builtin_amdgcn_alignbyte(an, a, 1), b1 = __builtin_amdgcn_alignbyte(bn, b, 1);
+            *reinterpret_cast<uint32_t*>(dp + (size_t)(r0 + y) * ds + c) = a;
+            *reinterpret_cast<uint32_t*>(dp + (size_t)(r0 + h + y) * ds + c) = avg8x4(a, a1);
+            *reinterpret_cast<uint32_t*>(dp + (size_t)(r0 + 2 * h + y) * ds + c) = avg8x4(a, b);
+            *reinterpret_cast<uint32_t*>(dp + (size_t)(r0 + 3 * h + y) * ds + c) = avg8x4(a, b1);
+        }
+    }
     for (int sub = 0; sub < nsub; sub++) {
         uint8_t* row = dp + (size_t)(g.level_row[plane][0] + sub * h + y) * ds;
         auto value = [&](int x) __attribute__((always_inline)) -> uint8_t {
@@ -127,7 +173,7 @@ __global__ void __launch_bounds__(kThreads) super_level0_kernel(L0Args a) {
             return 0;
         };
         if ((((uintptr_t)row) & 3) == 0) {
-            for (int c = threadIdx.x * 4; c < ds; c += kThreads * 4) {
+            for (int c = done + threadIdx.x * 4; c < ds; c += kThreads * 4) {
                 if (c + 4 <= ds) {
                     uint32_t wv = (uint32_t)value(c) | ((uint32_t)value(c + 1) << 8) | ((uint32_t)value(c + 2) << 16) |
                                   ((uint32_t)value(c + 3) << 24);
@@ -137,7 +183,7 @@ __global__ void __launch_bounds__(kThreads) super_level0_kernel(L0Args a) {
                 }
             }
         } else {
-            for (int c = threadIdx.x; c < ds; c += kThreads) row[c] = value(c);
+            for (int c = done + threadIdx.x; c < ds; c += kThreads) row[c] = value(c);
         }
     }
 }
