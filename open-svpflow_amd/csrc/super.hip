@@ -2,13 +2,15 @@
 //
 // Two kernels, both writing every byte of their rows (so the super frame needs no
 // clearing pass; the reference zeroes the plane first, super_filter.rs:210-223):
-//   super_level0_kernel  one workgroup per (source row y, plane, frame): stages the
-//                        six full-pel rows y-2..y+3 (edge-padded to the stride) in LDS
-//                        and writes row y of every level-0 subplane.
+//   super_level0_kernel  one workgroup per (kL0Rows source rows, plane, frame): stages
+//                        the full-pel rows y-2..y+3 of each (edge-padded to the stride)
+//                        in LDS and writes those rows of every level-0 subplane.
 //   super_reduce_kernel  one workgroup per (destination row, plane, frame) of level
 //                        l+1: vertical 6-tap into LDS, then the horizontal 6-tap with
 //                        the reference's first-tap rule, written over the full stride.
 // Rows are written as 32-bit words when the row start is 4-byte aligned, bytes otherwise.
+#include <algorithm>
+
 #include "svpf_internal.h"
 
 namespace svpf {
@@ -28,6 +30,7 @@ struct L0Args {
     SuperGeom g;
     const Planes* src;
     const Planes* dst;
+    int rows;  // source rows per workgroup, 1 .. kL0Rows (their staging fits 64 KiB of LDS)
 };
 
 // Full-pel region F(r, c) = src[r][min(c, w-1)] for c < stride (super_build.rs:45-67:
@@ -59,13 +62,16 @@ struct Rows {
     }
 };
 
+constexpr int kL0Rows = 4;  // most source rows per workgroup of super_level0_kernel
+
 __global__ void __launch_bounds__(kThreads) super_level0_kernel(L0Args a) {
-    const int y = blockIdx.x;
+    const int y0 = blockIdx.x * a.rows;
     const int plane = blockIdx.y;
     const int frame = blockIdx.z;
     const SuperGeom& g = a.g;
     const int w = g.level_w[plane][0], h = g.level_h[plane][0];
-    if (y >= h) return;
+    if (y0 >= h) return;
+    const int ny = min(a.rows, h - y0);
     // references into the descriptor array: a local copy indexed by `plane` would be
     // placed in scratch memory
     const Planes& src = a.src[frame];
@@ -77,18 +83,21 @@ __global__ void __launch_bounds__(kThreads) super_level0_kernel(L0Args a) {
 
     extern __shared__ uint8_t lds[];
     const int pitch = ds + 4;
-    // rows y-2..y+3 (clamped), edge-padded to the stride: the words inside the row with
-    // one (unaligned) 4-byte load per row, six in flight, then the padded tail by bytes
+    // rows y0-2 .. y0+ny+2 (clamped), edge-padded to the stride: the words inside the
+    // row with one (unaligned) 4-byte load per row, all rows' loads in flight, then the
+    // padded tail by bytes.  Output row y0 + j reads LDS rows j .. j+5.
+    const int nst = ny + 5;
     const int wfull = w & ~3;
     for (int c = threadIdx.x * 4; c < wfull; c += kThreads * 4) {
-        uint32_t v[6];
+        uint32_t v[kL0Rows + 5];
 #pragma unroll
-        for (int k = 0; k < 6; k++) {
-            const int r = min(max(y - 2 + k, 0), h - 1);
-            v[k] = *(const g_u32u*)((const g_u8*)sp + ((size_t)r * ss + c));
+        for (int k = 0; k < kL0Rows + 5; k++) {
+            const int r = min(max(y0 - 2 + k, 0), h - 1);
+            if (k < nst) v[k] = *(const g_u32u*)((const g_u8*)sp + ((size_t)r * ss + c));
         }
 #pragma unroll
-        for (int k = 0; k < 6; k++) {
+        for (int k = 0; k < kL0Rows + 5; k++) {
+            if (k >= nst) break;
             uint8_t* L = lds + k * pitch + c;
             L[0] = (uint8_t)v[k];
             L[1] = (uint8_t)(v[k] >> 8);
@@ -96,19 +105,21 @@ __global__ void __launch_bounds__(kThreads) super_level0_kernel(L0Args a) {
             L[3] = (uint8_t)(v[k] >> 24);
         }
     }
-    for (int k = 0; k < 6; k++) {
-        const int r = min(max(y - 2 + k, 0), h - 1);
+    for (int k = 0; k < nst; k++) {
+        const int r = min(max(y0 - 2 + k, 0), h - 1);
         const uint8_t* s = sp + (size_t)r * ss;
         for (int c = wfull + threadIdx.x; c < pitch; c += kThreads) lds[k * pitch + c] = s[min(c, w - 1)];
     }
     __syncthreads();
-    const Rows R{(lds_u8*)lds, sp, ss, pitch, y, h, w, ds};
-
     const int pel = g.pel;
     const int nsub = g.full ? pel * pel : 1;
     const bool fill = pel >= 2 && g.gpu <= 1 && g.full;
     const bool bilinear = fill && g.scale_up == 0;
     const bool bicubic = fill && g.scale_up == 2;
+    const bool aligned4 = ((((uintptr_t)dp) | (uintptr_t)ds | (uintptr_t)pitch) & 3) == 0;
+    for (int j = 0; j < ny; j++) {
+    const int y = y0 + j;
+    const Rows R{(lds_u8*)(lds + j * pitch), sp, ss, pitch, y, h, w, ds};
 
     // Bilinear pel 2 away from the edges (rows 1 .. h-2, pixels whose right neighbour
     // lies inside the row): the four subplanes four pixels at a time from words of rows y
@@ -116,12 +127,11 @@ __global__ void __launch_bounds__(kThreads) super_level0_kernel(L0Args a) {
     // F(y+1, x)), diagonal avg(F(y, x), F(y+1, x+1)) (super_build.rs:183-223 off its
     // first-row and past-the-end rules).  The rest of each row takes the loop below.
     int done = 0;  // columns [0, done) written here
-    const bool aligned4 = ((((uintptr_t)dp) | (uintptr_t)ds | (uintptr_t)pitch) & 3) == 0;
     if (bilinear && pel == 2 && aligned4 && y >= 1 && y + 1 < h) {
         done = (w - 1) & ~3;  // x + 4 < w for every x of a word below `done`
         const int r0 = g.level_row[plane][0];
-        const uint32_t* L0 = (const uint32_t*)(lds + 2 * pitch);  // row y (LDS row k = 2)
-        const uint32_t* L1 = (const uint32_t*)(lds + 3 * pitch);  // row y + 1
+        const uint32_t* L0 = (const uint32_t*)(lds + (j + 2) * pitch);  // row y
+        const uint32_t* L1 = (const uint32_t*)(lds + (j + 3) * pitch);  // row y + 1
         for (int c = threadIdx.x * 4; c < done; c += kThreads * 4) {
             const uint32_t a = L0[c >> 2], an = L0[(c >> 2) + 1], b = L1[c >> 2], bn = L1[(c >> 2) + 1];
             const uint32_t a1 = __builtin_amdgcn_alignbyte(an, a, 1), b1 = __builtin_amdgcn_alignbyte(bn, b, 1);
@@ -185,6 +195,7 @@ __global__ void __launch_bounds__(kThreads) super_level0_kernel(L0Args a) {
         } else {
             for (int c = done + threadIdx.x; c < ds; c += kThreads) row[c] = value(c);
         }
+    }
     }
 }
 
@@ -278,9 +289,11 @@ __global__ void __launch_bounds__(kThreads) super_zero_rows_kernel(SuperGeom g, 
 
 hipError_t launch_super_level0(const SuperGeom& g, const Planes* src, const Planes* dst, int nframes,
                                int max_stride, hipStream_t s) {
-    L0Args a{g, src, dst};
-    dim3 grid(g.level_h[0][0], 3, nframes);
-    size_t lds = 6 * (size_t)(max_stride + 4);
+    const int pitch = max_stride + 4;
+    const int rows = std::max(1, std::min(kL0Rows, 65536 / pitch - 5));
+    L0Args a{g, src, dst, rows};
+    dim3 grid((g.level_h[0][0] + rows - 1) / rows, 3, nframes);
+    size_t lds = (size_t)(rows + 5) * pitch;
     hipLaunchKernelGGL(super_level0_kernel, grid, dim3(kThreads), lds, s, a);
     return hipGetLastError();
 }
