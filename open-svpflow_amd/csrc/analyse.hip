@@ -1580,7 +1580,8 @@ __device__ __forceinline__ void queue_wave(const SweepState& st, int ci, const C
     if (take) st.wl[q][first + lane_rank(m)] = ((unsigned long long)ci << 32) | (unsigned)n;
 }
 
-constexpr int kScanThreads = 512;  // threads of a workgroup that rebuilds one chain's count
+constexpr int kScanThreads = 1024;  // threads of a workgroup that rebuilds one chain's count
+constexpr int kScanRegs = 32;       // a chunk of at most this many positions is held in registers
 
 // Exclusive prefix sum of v over the workgroup's threads (wsum: one int per wave).
 __device__ __forceinline__ int block_excl_scan(int v, int* wsum) {
@@ -1654,6 +1655,61 @@ __device__ void scan_chain(const Chain* __restrict__ chains, int ci, const Sweep
     }
     int* wsum = sh + T;
     int in = guess;
+    // Chunks of at most kScanRegs positions (every level of up to kScanThreads *
+    // kScanRegs blocks) are walked from registers, loaded once; longer ones from memory.
+    // (m is the same for every thread, so is the branch.)
+    if (m <= kScanRegs) {
+        uint32_t fr[kScanRegs];
+#pragma unroll
+        for (int j = 0; j < kScanRegs; j++) fr[j] = lo + j < hi ? fs[lo + j] : 0u;  // 0: never counts
+        auto walk = [&](int c) {
+#pragma unroll
+            for (int j = 0; j < kScanRegs; j++)
+                if ((int)fr[j] > bad_limit(g.bad_sad, c)) c++;
+            return c;
+        };
+        for (int it = 0; it <= T; it++) {  // exact after T iterations at the latest
+            const int d = walk(in) - in;
+            const int v = c0 + block_excl_scan(d, wsum);
+            const int changed = v != in;
+            in = v;
+            if (!__syncthreads_or(changed)) break;
+        }
+        int c = in;
+#pragma unroll
+        for (int j0 = 0; j0 < kScanRegs; j0 += kScanBatch) {
+            const int k0 = lo + j0;
+            if (k0 >= hi) break;
+            uint32_t b[kScanBatch];
+            int cc[kScanBatch];
+#pragma unroll
+            for (int j = 0; j < kScanBatch; j++) {
+                const bool v = k0 + j < hi;
+                b[j] = v ? bs[k0 + j] : 0u;
+                cc[j] = v ? cs[k0 + j] : c;
+            }
+#pragma unroll
+            for (int j = 0; j < kScanBatch; j++) {
+                const int k = k0 + j;
+                const bool v = k < hi;
+                const int bj = (int)fr[j0 + j] > bad_limit(g.bad_sad, c) ? 1 : 0;
+                const bool cchg = v && cc[j] != c, want = v && (uint32_t)bj != b[j];
+                int idx = 0;
+                if (cchg || want) {
+                    const int pk = ch.order[k];
+                    idx = (pk >> 16) * g.gw + (pk & 0xFFFF);
+                }
+                if (cchg) {
+                    st.cpre[ch.base + idx] = c;
+                    cs[k] = c;
+                }
+                queue_wave(st, ci, ch, idx, want, s);
+                if (v) c += bj;
+            }
+        }
+        if (tid == 0) st.minpos[ci] = 0xFFFFFFFFu;
+        return;
+    }
     for (int it = 0; it <= T; it++) {  // exact after T iterations at the latest
         const int d = scan_walk(fs, lo, hi, g.bad_sad, in) - in;
         const int v = c0 + block_excl_scan(d, wsum);
