@@ -413,19 +413,21 @@ __device__ __forceinline__ uint32_t tile_cost(const uint32_t src[4], const g_u8*
 }
 
 // The candidate's (luma, 4 x chroma) cost through the plan; every lane of the group
-// receives it (the same values as group_cost).
+// receives it (the same values as group_cost).  Rows, subplane indexes and strides are
+// below 2^16 and a covered candidate's row is not negative, so their products are
+// exact as 24-bit multiplies (full rate; a 32-bit multiply is quarter rate).
 __device__ __forceinline__ void plan_cost(const TilePlan& P, int mvx, int mvy, uint32_t& L, uint32_t& C) {
-    const int sub = P.subpel ? (mvy & 1) * P.pel + (mvx & 1) : 0;
+    const int sub = P.subpel ? __mul24(mvy & 1, P.pel) + (mvx & 1) : 0;
     uint32_t hl = 0, hc = 0;
     if (P.luse) {
         const int sh = P.subpel;  // luma vectors are in half pels when sub-pel
-        const int row = P.lrow0 + sub * P.lh + (mvy >> sh), col = P.lcol0 + (mvx >> sh);
-        hl = tile_cost(P.lsrc, P.lbase + (row * P.lrs + col), P.lrs, P.satd != 0);
+        const int row = P.lrow0 + __mul24(sub, P.lh) + (mvy >> sh), col = P.lcol0 + (mvx >> sh);
+        hl = tile_cost(P.lsrc, P.lbase + (__mul24(row, P.lrs) + col), P.lrs, P.satd != 0);
     }
     if (P.cuse) {  // (uniform: the whole group takes part in the pair exchange)
         const int csh = P.subpel + 1;
-        const int row = P.crow0 + sub * P.clh + (mvy >> csh), col = P.ccol0 + (mvx >> csh);
-        const g_u8* r = P.cbase + (row * P.crs + col);
+        const int row = P.crow0 + __mul24(sub, P.clh) + (mvy >> csh), col = P.ccol0 + (mvx >> csh);
+        const g_u8* r = P.cbase + (__mul24(row, P.crs) + col);
         const uint32_t y[2] = {gld32(r), gld32(r + P.crs)};
         if (P.satd) {
             hc = had4_half_pk(P.csrc, y, (threadIdx.x & 1) == 0);
@@ -457,10 +459,13 @@ __device__ __forceinline__ bool in_bounds(const Search& s, int x, int y) {
     return x >= s.bnd[0] && y >= s.bnd[1] && x < s.bnd[2] && y < s.bnd[3];
 }
 
-// motion_penalty (:1263-1267) of vector (x, y) against the search's predictor.
+// motion_penalty (:1263-1267) of vector (x, y) against the search's predictor.  Vector
+// components are 16-bit, so |dx|, |dy| < 2^16 and each square (below 2^32, as an
+// unsigned sum below 2^33) is exact from a 24-bit multiply's 48-bit product.
 __device__ __forceinline__ int64_t motion_of(const Search& s, int x, int y) {
-    const int64_t dx = s.pmx - x, dy = s.pmy - y;
-    return ((int64_t)s.lambda * (dx * dx + dy * dy)) >> 8;
+    const int dx = s.pmx - x, dy = s.pmy - y;
+    const uint64_t d2 = (uint64_t)(uint32_t)__mul24(dx, dx) + (uint64_t)(uint32_t)__mul24(dy, dy);
+    return ((int64_t)s.lambda * (int64_t)d2) >> 8;
 }
 
 // exact_check's first two tests (:1122-1155): inside the vector bounds, and a motion
@@ -615,8 +620,8 @@ __device__ __forceinline__ void pat_gen_kind(int kind, const Pattern& p, int k, 
         } else {
             const unsigned j = uk - 4 * nd, h = j & 15u;
             const int sc = 1 + (int)(j >> 4);
-            x = cx + ((int)((kHex4X >> (4 * h)) & 15u) - 4) * sc;
-            y = cy + ((int)((kHex4Y >> (4 * h)) & 15u) - 4) * sc;
+            x = cx + __mul24((int)((kHex4X >> (4 * h)) & 15u) - 4, sc);  // |.| <= 4 times a small scale
+            y = cy + __mul24((int)((kHex4Y >> (4 * h)) & 15u) - 4, sc);
         }
     } else {  // kPatExh: k = row * w + column, the quotient by a multiply (k, w < 2^16)
         const unsigned w = 2u * (unsigned)r + 1u;
