@@ -361,13 +361,15 @@ struct Src {
     int stride, span, shift;
 };
 
+// A plane (or the pel^2 subplanes of a super plane) is below 2^32 bytes and its rows and
+// strides below 2^16, so the offset is 32-bit arithmetic from the plane's base.
 __device__ __forceinline__ int sample(const Src& s, const PP& p, int sx, int sy, int dx, int dy) {
     const int x = clampi(sx + dx, 0, p.max_x), y = clampi(sy + dy, 0, p.max_y);
     gm_u8* d = (gm_u8*)s.d;
-    if (s.shift == 0) return d[(size_t)y * s.stride + x];
+    if (s.shift == 0) return d[__umul24((uint32_t)y, (uint32_t)s.stride) + (uint32_t)x];
     const int mask = (1 << s.shift) - 1;
-    const int sub = ((y & mask) << s.shift) | (x & mask);
-    return d[(size_t)sub * s.span + (size_t)(y >> s.shift) * s.stride + (x >> s.shift)];
+    const uint32_t sub = (uint32_t)(((y & mask) << s.shift) | (x & mask));
+    return d[sub * (uint32_t)s.span + __umul24((uint32_t)(y >> s.shift), (uint32_t)s.stride) + (uint32_t)(x >> s.shift)];
 }
 
 // One thread per (plane row, tile) of one frame: the pixels of a tile row share their
@@ -420,9 +422,7 @@ __global__ void __launch_bounds__(256) render_kernel(const FrameJob* __restrict_
     if (gx < 0) {
         xb = 0; xe = min(p.ox, pw); bx = 0;
     } else {
-        if (p.ox >= pw) return;
-        const int ext = max((pw - p.ox + p.bw - 1) / p.bw - 1, 0);  // first tile reaching the plane end
-        const int last = min(mw - 1, ext);
+        const int last = g.tile_last_x[chroma];  // the first tile reaching the plane end (-1: none)
         if (gx > last) return;
         bx = p.ox + p.bw * gx;
         xb = bx;
@@ -432,9 +432,9 @@ __global__ void __launch_bounds__(256) render_kernel(const FrameJob* __restrict_
     int gy, by;
     if (py < p.oy) {
         gy = -1; by = 0;
-    } else {
-        const int ext = max((ph - p.oy + p.bh - 1) / p.bh - 1, 0);
-        gy = min((py - p.oy) / p.bh, min(mh - 1, ext));
+    } else {  // (py - oy) / bh as a multiply-high: exact for operands below 2^16
+        const uint32_t q = g.rcp_bh[chroma] ? __umulhi((uint32_t)(py - p.oy), g.rcp_bh[chroma]) : (uint32_t)(py - p.oy);
+        gy = min((int)q, g.tile_last_y[chroma]);
         by = p.oy + p.bh * gy;
     }
     const int x0 = max(gx, 0), y0 = max(gy, 0);

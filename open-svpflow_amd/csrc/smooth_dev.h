@@ -23,6 +23,11 @@ struct SmoothGeom {
     int32_t xshift[2], yshift[2];    // cpu_map of the block (luma) / half block (chroma)
     int32_t xw[2][32][2], yw[2][32][2];  // interpolation weights, luma / chroma
     int32_t interp;                  // !block
+    // render tiles per plane kind (luma, chroma): the last tile column and row (the first
+    // tile reaching the plane end; -1: the origin lies past the plane, no tiles), and
+    // ceil(2^32 / tile height) so that a row's tile is a multiply-high (0: height 1)
+    int32_t tile_last_x[2], tile_last_y[2];
+    uint32_t rcp_bh[2];
     // options
     int32_t algo_req, force13, mask_area, mask_cover;
     double area_scale, area_sharp;

@@ -483,6 +483,14 @@ int svpf_smooth_create(svpf_ctx* ctx, const char* opt, size_t opt_len, int32_t w
     weights(g.cbw / 2, g.xshift[1], g.xw[1]);
     weights(g.cbh / 2, g.yshift[1], g.yw[1]);
     g.interp = !s->opt.block;
+    for (int c = 0; c < 2; c++) {  // render_tiles (renderer.rs:1925-2010) per plane kind, as render_kernel reads them
+        const int pw = c ? width / 2 : width, ph = c ? height / 2 : height;
+        const int bw = c ? g.cbw / 2 : g.cbw, bh = c ? g.cbh / 2 : g.cbh;
+        const int ox = g.interp ? (c ? g.cox / 2 : g.cox) : 0, oy = g.interp ? (c ? g.coy / 2 : g.coy) : 0;
+        g.tile_last_x[c] = (bw > 0 && ox < pw) ? std::min(g.mw - 1, std::max((pw - ox + bw - 1) / bw - 1, 0)) : -1;
+        g.tile_last_y[c] = (bh > 0 && oy < ph) ? std::min(g.mh - 1, std::max((ph - oy + bh - 1) / bh - 1, 0)) : -1;
+        g.rcp_bh[c] = bh > 1 ? (uint32_t)((((uint64_t)1 << 32) + (uint64_t)bh - 1) / (uint64_t)bh) : 0u;
+    }
     g.algo_req = (int)s->opt.algo;
     g.force13 = s->opt.force13;
     g.mask_area = s->opt.area > 0;
