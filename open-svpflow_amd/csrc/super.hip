@@ -224,7 +224,33 @@ __global__ void __launch_bounds__(kThreads) super_reduce_kernel(ReduceArgs a) {
     extern __shared__ uint8_t inter[];  // src_w bytes: the vertically filtered row
     const int sy = dy * 2;
     const bool use_edge = dy == 0 || dy + 1 == dst_h || sy - 2 < 0 || sy + 3 >= src_h;
-    for (int x = threadIdx.x; x < src_w; x += kThreads) {
+    // four columns per thread from 4-byte loads: the 6-tap in two 16-bit lanes (even and
+    // odd bytes; a lane's sum is at most 255 * 32 + 16 = 8176, so the reference's
+    // min(255, .) never applies), the edge rows' average bytewise; the rest below
+    const int wq = src_w & ~3;
+    for (int x = threadIdx.x * 4; x < wq; x += kThreads * 4) {
+        uint32_t v;
+        if (use_edge) {
+            const int y0 = min(sy, src_h - 1), y1 = min(sy + 1, src_h - 1);
+            v = avg8x4(*(const g_u32u*)((const g_u8*)srow0 + ((size_t)y0 * ds + x)),
+                       *(const g_u32u*)((const g_u8*)srow0 + ((size_t)y1 * ds + x)));
+        } else {
+            const g_u8* c = (const g_u8*)srow0 + ((size_t)(sy - 2) * ds + x);
+            uint32_t q[6];
+#pragma unroll
+            for (int k = 0; k < 6; k++) q[k] = *(const g_u32u*)(c + (size_t)k * ds);
+            auto tap = [&](int sh) {
+                uint32_t e[6];
+#pragma unroll
+                for (int k = 0; k < 6; k++) e[k] = (q[k] >> sh) & 0x00FF00FFu;
+                const uint32_t t = e[0] + e[5] + 5u * (e[1] + e[4]) + 10u * (e[2] + e[3]) + 0x00100010u;
+                return ((t >> 5) & 0x00FF00FFu) << sh;
+            };
+            v = tap(0) | tap(8);
+        }
+        *reinterpret_cast<uint32_t*>(inter + x) = v;
+    }
+    for (int x = wq + threadIdx.x; x < src_w; x += kThreads) {
         uint8_t v;
         if (use_edge) {
             int y0 = min(sy, src_h - 1), y1 = min(sy + 1, src_h - 1);
