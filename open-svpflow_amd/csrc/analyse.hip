@@ -1895,39 +1895,35 @@ __global__ void __launch_bounds__(kWave, kFast ? 3 : 2) sweep0_kernel(const Chai
 // whose recorded neighbour vectors differ from the neighbours' results now (every
 // other input of a block off the edge row and column is fixed for the level; a changed
 // bad-block count is handled by the count rebuild), and every edge block (no record).
-// One thread per block, blocks in raster order; a thread issues all its loads at once
+// One thread per block (grid y: the chain), blocks in raster order; a thread issues all its loads at once
 // (the record as three 16-byte words, each neighbour's visit position and vector).
-__global__ void __launch_bounds__(256) validate_kernel(const Chain* __restrict__ chains, SweepState st, int max_n,
-                                                       long long total) {
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
-         t += (long long)gridDim.x * blockDim.x) {
-        const int ci = (int)(t / max_n), idx0 = (int)(t - (long long)ci * max_n);
-        const Chain& ch = chains[ci];
-        const LevelGeom& g = ch.geom;
-        const int gw = g.gw, gh = g.gh;
-        const bool here = idx0 < gw * gh;
-        const int idx = here ? idx0 : 0;
-        const int bx = idx % gw, by = idx / gw;
-        const bool edge = g.level > 0 && (bx == gw - 1 || by == gh - 1);
-        const uint4* rw = reinterpret_cast<const uint4*>(&st.seeds[ch.base + idx]);
-        const uint4 r0 = rw[0], r1 = rw[1], r2 = rw[2];
-        const uint32_t seed[9] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x};
-        const int me = ch.opos[idx];
-        int pos[9];
-        uint32_t vec[9];
+__global__ void __launch_bounds__(256) validate_kernel(const Chain* __restrict__ chains, SweepState st) {
+    const int ci = blockIdx.y, idx0 = blockIdx.x * blockDim.x + threadIdx.x;  // (chain, raster index)
+    const Chain& ch = chains[ci];
+    const LevelGeom& g = ch.geom;
+    const int gw = g.gw, gh = g.gh;
+    const bool here = idx0 < gw * gh;
+    const int idx = here ? idx0 : 0;
+    const int bx = idx % gw, by = idx / gw;
+    const bool edge = g.level > 0 && (bx == gw - 1 || by == gh - 1);
+    const uint4* rw = reinterpret_cast<const uint4*>(&st.seeds[ch.base + idx]);
+    const uint4 r0 = rw[0], r1 = rw[1], r2 = rw[2];
+    const uint32_t seed[9] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x};
+    const int me = ch.opos[idx];
+    int pos[9];
+    uint32_t vec[9];
 #pragma unroll
-        for (int q = 0; q < 9; q++) {
-            const int nx = bx - 1 + q % 3, ny = by - 1 + q / 3;
-            const bool in = q != 4 && nx >= 0 && ny >= 0 && nx < gw && ny < gh;
-            const int n = in ? ny * gw + nx : idx;
-            pos[q] = in ? ch.opos[n] : INT32_MAX;
-            vec[q] = *reinterpret_cast<const uint32_t*>(&ch.res[n]);
-        }
-        bool stale = edge || r2.z == 0;  // no record at the edge; SeedRec::valid
-#pragma unroll
-        for (int q = 0; q < 9; q++) stale = stale || (pos[q] < me && vec[q] != seed[q]);
-        queue_wave(st, ci, ch, idx, here && stale, 0);
+    for (int q = 0; q < 9; q++) {
+        const int nx = bx - 1 + q % 3, ny = by - 1 + q / 3;
+        const bool in = q != 4 && nx >= 0 && ny >= 0 && nx < gw && ny < gh;
+        const int n = in ? ny * gw + nx : idx;
+        pos[q] = in ? ch.opos[n] : INT32_MAX;
+        vec[q] = *reinterpret_cast<const uint32_t*>(&ch.res[n]);
     }
+    bool stale = edge || r2.z == 0;  // no record at the edge; SeedRec::valid
+#pragma unroll
+    for (int q = 0; q < 9; q++) stale = stale || (pos[q] < me && vec[q] != seed[q]);
+    queue_wave(st, ci, ch, idx, here && stale, 0);
 }
 
 // Claims block idx for an evaluation in sweep s (s >= 1): false when another wave of
@@ -2492,9 +2488,7 @@ hipError_t launch_deferred(const Chain* d_chains, int grid, const SweepState& st
 
 
 hipError_t launch_validate(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s) {
-    const long long total = (long long)nchains * max_n;
-    const unsigned grid = (unsigned)std::min<long long>((total + 255) / 256, 65536);
-    hipLaunchKernelGGL(validate_kernel, dim3(grid), dim3(256), 0, s, d_chains, st, max_n, total);
+    hipLaunchKernelGGL(validate_kernel, dim3((max_n + 255) / 256, nchains), dim3(256), 0, s, d_chains, st);
     return hipGetLastError();
 }
 
