@@ -1,7 +1,7 @@
 """svpf_analyse_pair_noalloc (the allocation-free form of svpf_analyse_pair) through the
-C ABI: its payload equals svpf_analyse_pair's byte for byte on two calls in a row, the
-second on a smaller frame pair (the shared scratch array keeps the first call's
-contents), and the smaller one equals the oracle; a pair larger than the scratch array
+C ABI: its payload equals svpf_analyse_pair's byte for byte on calls in a row, each on a
+smaller frame pair than the last (the shared scratch array keeps the previous call's
+contents), starting at the largest documented size, and the smallest equals the oracle; a pair larger than the scratch array
 is rejected with an error that says so."""
 import ctypes as C
 
@@ -43,7 +43,9 @@ def _supers(ctx, frames, sopt):
 def test_noalloc_equals_analyse_pair(gpu_ctx, oracle):
     ctx = gpu_ctx
     fn = _noalloc()
-    for w, h, aopt, seed in ((1280, 720, CFG2, 71), (320, 192, SMALL, 72)):
+    # the largest pair the scratch array is documented to hold (2560x1440, 8x8 blocks,
+    # overlap 4), then smaller ones
+    for w, h, aopt, seed in ((2560, 1440, CFG2, 70), (1280, 720, CFG2, 71), (320, 192, SMALL, 72)):
         print(f"noalloc: {w}x{h} {aopt}", flush=True)
         frames = synth.clip(w, h, 2, seed=seed, vmax=8.0, nobj=(2, 3), rot_deg=1.0)
         sdata, sup = _supers(ctx, frames, "{pel:2}")
