@@ -2,12 +2,15 @@
 //
 // Two kernels, both writing every byte of their rows (so the super frame needs no
 // clearing pass; the reference zeroes the plane first, super_filter.rs:210-223):
-//   super_level0_kernel  one workgroup per (kL0Rows source rows, plane, frame): stages
-//                        the full-pel rows y-2..y+3 of each (edge-padded to the stride)
-//                        in LDS and writes those rows of every level-0 subplane.
+//   super_level0_kernel  one workgroup per (up to kL0Rows source rows, plane, frame):
+//                        stages the full-pel rows y-2..y+3 of each (edge-padded to the
+//                        stride) in LDS with 4-byte loads and writes those rows of every
+//                        level-0 subplane (bilinear pel-2 interiors four pixels at a time).
+//                        Fewer rows where their staging would exceed 64 KiB of LDS.
 //   super_reduce_kernel  one workgroup per (destination row, plane, frame) of level
-//                        l+1: vertical 6-tap into LDS, then the horizontal 6-tap with
-//                        the reference's first-tap rule, written over the full stride.
+//                        l+1: vertical 6-tap into LDS (four columns per thread), then the
+//                        horizontal 6-tap with the reference's first-tap rule, written
+//                        over the full stride.
 // Rows are written as 32-bit words when the row start is 4-byte aligned, bytes otherwise.
 #include <algorithm>
 
