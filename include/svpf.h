@@ -170,7 +170,11 @@ int svpf_analyse_set_max_sweeps(svpf_ctx* ctx, int32_t max_sweeps);
  *   coarse_run   the same at the coarser levels (4, 8, 16 or 32; default 16);
  *   graphs       1 (default): the later sweeps of a level are replayed from a captured
  *                hipGraph of six sweeps (cached per launch configuration); -1: launched
- *                one by one.  The chase and svpf_analyse_pair_noalloc never use graphs. */
+ *                one by one.  The chase and svpf_analyse_pair_noalloc never use graphs;
+ *   fast_list_min at a coarse level whose sweep 0 ran the straight-line tile-plan kernel, a
+ *                later sweep whose list holds at least this many blocks runs that kernel
+ *                too, with a deferred pass for the blocks it passes on (default 4096;
+ *                -1: never). */
 typedef struct svpf_analyse_tuning {
     int32_t chase;
     double slot_scale;
@@ -179,6 +183,7 @@ typedef struct svpf_analyse_tuning {
     int32_t fine_run;
     int32_t coarse_run;
     int32_t graphs;
+    int32_t fast_list_min;
 } svpf_analyse_tuning;
 int svpf_analyse_set_tuning(svpf_ctx* ctx, const svpf_analyse_tuning* t);
 

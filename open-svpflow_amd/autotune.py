@@ -62,7 +62,7 @@ def apply(ctx: svpf.Context, setting: Dict[str, float]) -> None:
     s.update(setting)
     ctx.set_tuning(slot_scale=float(s["slot_scale"]), coarse_slots=int(s["coarse_slots"]),
                    sweep_grid=int(s["sweep_grid"]), fine_run=int(s["fine_run"]), coarse_run=int(s["coarse_run"]),
-                   graphs=int(s.get("graphs", 0)))
+                   graphs=int(s.get("graphs", 0)), fast_list_min=int(s.get("fast_list_min", 0)))
 
 
 def descend(measure: Callable[[Dict[str, float]], float], space: Dict[str, Sequence] = SPACE,

@@ -1791,7 +1791,9 @@ __device__ __forceinline__ bool sweep_item(const Chain* __restrict__ chains, int
                                               st.seeds + ch.base + idx, &unchanged, RefineStart{}, &defer, &io);
     if (kFast && defer != kDeferNone) {
         if (lane_id() == 0) st.dl[atomicAdd(&st.dcount[kDcDeferred], 1u)] = ((unsigned long long)ci << 32) | (unsigned)idx;
-        if (defer == kDeferWhole) return false;
+        // in a later sweep a provisional result is not stored either: storing it would queue
+        // the block's neighbours twice (now and when the deferred search changes it again)
+        if (defer == kDeferWhole || s > 0) return false;
     }
     if (unchanged) {  // nothing it reads changed since its last evaluation
         if (lane_id() == 0) atomicAdd(&st.counts[kCntSkipped], 1u);  // skip count (statistics)
@@ -1956,8 +1958,14 @@ __device__ __forceinline__ int order_successor(const Chain* __restrict__ chains,
 // changes after that queues it for the next sweep.  (The lists are short after the first
 // few sweeps and each sweep waits on its slowest wave, so a second, deferred pass per
 // sweep would cost more than three waves per SIMD gain.)
-template <bool kL0>
-__global__ void __launch_bounds__(kWave, 2) sweep_kernel(const Chain* __restrict__ chains, SweepState st, int s) {
+//
+// kFast (with kLean): the coarse levels' tile-plan instantiation, for the long lists of
+// the first sweeps after sweep 0; a block it cannot finish stores nothing and goes to the
+// deferred list, which deferred_kernel completes before sweep s + 1 (the host clears
+// that list's length first).  No chase.
+template <bool kL0, bool kFast = false, bool kLean = false>
+__global__ void __launch_bounds__(kWave, kFast ? 3 : 2) sweep_kernel(const Chain* __restrict__ chains, SweepState st,
+                                                                     int s) {
     const int p = s & 1;
     // this workgroup's first item, loaded with the list length rather than after it (the
     // grid is at most one workgroup per block of the level, so the slot exists)
@@ -1973,8 +1981,8 @@ __global__ void __launch_bounds__(kWave, 2) sweep_kernel(const Chain* __restrict
         // uniform across the wave: keeps the chain's descriptors in scalar registers
         const int ci = __builtin_amdgcn_readfirstlane((int)(item >> 32));
         int idx = __builtin_amdgcn_readfirstlane((int)(item & 0xFFFFFFFFu));
-        if (chase == 0) {  // every queued block once (its flag), no claim needed
-            sweep_item<false, kL0>(chains, ci, idx, state_here(), s, nullptr, p);
+        if (kFast || chase == 0) {  // every queued block once (its flag), no claim needed
+            sweep_item<kFast, kL0, kLean>(chains, ci, idx, state_here(), s, nullptr, p);
             continue;
         }
         const uint32_t base = (uint32_t)uniform_int((int)chains[ci].base);
@@ -2472,6 +2480,11 @@ hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, i
         hipLaunchKernelGGL(sweep_kernel<true>, dim3(grid), dim3(kWave), 0, s, d_chains, st, sweep);
     else
         hipLaunchKernelGGL(sweep_kernel<false>, dim3(grid), dim3(kWave), 0, s, d_chains, st, sweep);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_fast(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s) {
+    hipLaunchKernelGGL((sweep_kernel<false, true, true>), dim3(grid), dim3(kWave), 0, s, d_chains, st, sweep);
     return hipGetLastError();
 }
 

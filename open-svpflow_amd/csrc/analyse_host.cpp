@@ -16,6 +16,7 @@ hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const Sw
                          int level, bool fast, bool l0, bool lean, double slot_scale, int coarse_slots, int fine_run,
                          int coarse_run);
 hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s, bool l0);
+hipError_t launch_sweep_fast(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s);
 hipError_t launch_sweep_stats(const SweepState& st, int sweeps, long long total_blocks, unsigned long long* out4,
                               hipStream_t s);
 hipError_t launch_deferred(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s);
@@ -155,6 +156,7 @@ constexpr int kMaxSweeps = 4096;
 constexpr int kNotConverged = 1;
 
 
+constexpr int kFastListMin = 4096;  // default of svpf_analyse_tuning::fast_list_min
 constexpr int kGraphSweeps = 6;  // sweeps per captured graph: lists by s mod 2, queue lengths by s mod 3
 constexpr size_t kSweepGraphs = 16;
 
@@ -249,7 +251,39 @@ int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total
         const int rc = sweep_graph(ctx, dch, nch, grid, st, scan, l0, &exec);
         if (rc) return rc;
     }
-    int s = 1, chunk = exec ? kGraphSweeps : 8;
+    int s = 1;
+    auto converged = [&]() -> int {
+        if (ctx->prof)  // statistics for the benchmark, added on the device
+            SVPF_HIP(launch_sweep_stats(st, s, (long long)total_blocks, ctx->d_sweep_stats + 4 * std::min(level, 15),
+                                        ctx->stream));
+        return SVPF_OK;
+    };
+    // Coarse levels whose sweep 0 ran the straight-line tile-plan kernel: while a list
+    // holds at least fast_list_min blocks, the later sweep runs that kernel too (its
+    // general instantiation takes about three times as long per block there), and
+    // deferred_kernel completes the blocks it passed on; then single launches of the
+    // general kernel up to the next graph boundary.
+    if (fast && lean && !l0 && st.chase == 0 && ctx->fast_list_min > 0) {
+        for (;;) {
+            SVPF_HIP(hipMemcpyAsync(h_count, st.counts + s % 3, 4, hipMemcpyDeviceToHost, ctx->stream));
+            SVPF_HIP(hipStreamSynchronize(ctx->stream));
+            if (*h_count == 0) return converged();
+            if (*h_count < (uint32_t)ctx->fast_list_min) break;
+            if (s + 1 > st.max_sweeps) return kNotConverged;
+            SVPF_HIP(hipMemsetAsync(st.dcount + kDcDeferred, 0, 4, ctx->stream));
+            SVPF_HIP(launch_sweep_fast(dch, grid, st, s, ctx->stream));
+            SVPF_HIP(launch_deferred(dch, grid, st, s, ctx->stream));
+            if (scan) SVPF_HIP(launch_sweep_scan(dch, nch, st, s, ctx->stream));
+            s++;
+        }
+        while (exec && (s - 1) % kGraphSweeps != 0) {
+            if (s + 1 > st.max_sweeps) return kNotConverged;
+            SVPF_HIP(launch_sweep(dch, grid, st, s, ctx->stream, l0));
+            if (scan) SVPF_HIP(launch_sweep_scan(dch, nch, st, s, ctx->stream));
+            s++;
+        }
+    }
+    int chunk = exec ? kGraphSweeps : 8;
     for (;;) {
         if (s + chunk > st.max_sweeps) return kNotConverged;
         if (exec) {
@@ -263,12 +297,7 @@ int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total
         s += chunk;
         SVPF_HIP(hipMemcpyAsync(h_count, st.counts + s % 3, 4, hipMemcpyDeviceToHost, ctx->stream));
         SVPF_HIP(hipStreamSynchronize(ctx->stream));
-        if (*h_count == 0) {
-            if (ctx->prof)  // statistics for the benchmark, added on the device
-                SVPF_HIP(launch_sweep_stats(st, s, (long long)total_blocks,
-                                            ctx->d_sweep_stats + 4 * std::min(level, 15), ctx->stream));
-            return SVPF_OK;
-        }
+        if (*h_count == 0) return converged();
         chunk = std::min(chunk * 2, exec ? 10 * kGraphSweeps : 64);
     }
 }
@@ -971,7 +1000,7 @@ int svpf_analyse_set_tuning(svpf_ctx* ctx, const svpf_analyse_tuning* t) {
         return fail(SVPF_ERR_ARG, "svpf_analyse_set_tuning: coarse_run must be 4, 8, 16 or 32");
     if (t->chase > 65536 || t->slot_scale < 0.0 || t->slot_scale > 16.0 || t->coarse_slots < 0 ||
         t->coarse_slots > (1 << 20) || t->sweep_grid < 0 || t->sweep_grid > (1 << 20) || t->graphs < -1 ||
-        t->graphs > 1)
+        t->graphs > 1 || t->fast_list_min < -1)
         return fail(SVPF_ERR_ARG, "svpf_analyse_set_tuning: setting out of range");
     ctx->chase = std::max(t->chase, 0);
     ctx->slot_scale = t->slot_scale == 0.0 ? 1.0 : t->slot_scale;
@@ -980,6 +1009,7 @@ int svpf_analyse_set_tuning(svpf_ctx* ctx, const svpf_analyse_tuning* t) {
     ctx->coarse_run = t->coarse_run == 0 ? 16 : t->coarse_run;
     ctx->sweep_grid = t->sweep_grid == 0 ? 2048 : t->sweep_grid;
     ctx->sweep_graphs = t->graphs >= 0;
+    ctx->fast_list_min = t->fast_list_min == 0 ? kFastListMin : t->fast_list_min;
     return SVPF_OK;
 }
 

@@ -28,7 +28,7 @@ class SvpfError(RuntimeError):
 class AnalyseTuning(C.Structure):
     _fields_ = [("chase", C.c_int32), ("slot_scale", C.c_double), ("coarse_slots", C.c_int32),
                 ("sweep_grid", C.c_int32), ("fine_run", C.c_int32), ("coarse_run", C.c_int32),
-                ("graphs", C.c_int32)]
+                ("graphs", C.c_int32), ("fast_list_min", C.c_int32)]
 
 
 class Frame(C.Structure):
@@ -191,10 +191,11 @@ class Context:
         check(lib().svpf_analyse_set_max_sweeps(self.h, n))
 
     def set_tuning(self, chase: int = 0, slot_scale: float = 0.0, coarse_slots: int = 0, sweep_grid: int = 0,
-                   fine_run: int = 0, coarse_run: int = 0, graphs: int = 0):
+                   fine_run: int = 0, coarse_run: int = 0, graphs: int = 0, fast_list_min: int = 0):
         """svpf_analyse_set_tuning: zero keeps a default (chase 0: no chase; graphs -1: later
-        sweeps launched one by one instead of replayed from a captured hipGraph)."""
-        t = AnalyseTuning(chase, slot_scale, coarse_slots, sweep_grid, fine_run, coarse_run, graphs)
+        sweeps launched one by one instead of replayed from a captured hipGraph;
+        fast_list_min -1: no coarse later sweep runs the tile-plan kernel)."""
+        t = AnalyseTuning(chase, slot_scale, coarse_slots, sweep_grid, fine_run, coarse_run, graphs, fast_list_min)
         check(lib().svpf_analyse_set_tuning(self.h, C.byref(t)))
 
     def analyse_stats(self, level: int):

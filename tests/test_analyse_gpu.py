@@ -160,7 +160,9 @@ def test_analyse_sequential_fallback(gpu_ctx, oracle, max_sweeps):
                                     dict(coarse_run=4, coarse_slots=64), dict(coarse_run=16, coarse_slots=4096),
                                     dict(fine_run=32, slot_scale=2.8),
                                     dict(slot_scale=0.25, coarse_slots=512), dict(slot_scale=4.0, sweep_grid=64),
-                                    dict(graphs=-1), dict(graphs=-1, sweep_grid=128)])
+                                    dict(graphs=-1), dict(graphs=-1, sweep_grid=128),
+                                    dict(fast_list_min=1), dict(fast_list_min=1, graphs=-1, sweep_grid=64),
+                                    dict(fast_list_min=-1)])
 def test_analyse_tuning_keeps_vectors(gpu_ctx, oracle, tuning):
     """The search's schedule settings (chase along the visit order, slot counts, sweep
     grid) change only how the level's fixed point is reached, not the vectors."""
@@ -169,6 +171,25 @@ def test_analyse_tuning_keeps_vectors(gpu_ctx, oracle, tuning):
     gpu_ctx.set_tuning(**tuning)
     try:
         _check(gpu_ctx, oracle, frames, "{pel:2}", CFG2, batch=True)
+    finally:
+        gpu_ctx.set_tuning()
+
+
+@pytest.mark.parametrize("aopt", [
+    CFG2,
+    "{block:{w:8,h:8,overlap:2}, main:{levels:3, search:{type:3,distance:8,coarse:{bad:{sad:200}}}}}",
+    "{block:{w:4,h:4,overlap:0}, main:{search:{type:2,distance:6,coarse:{bad:{range:-3, sad:40}}}}}",
+    "{block:{w:8,h:8,overlap:3}, main:{search:{type:2,distance:8,coarse:{distance:-6, satd:false}}}}",
+])
+def test_analyse_fast_coarse_later_sweeps(gpu_ctx, oracle, aopt):
+    """Every coarse-level later sweep through the tile-plan kernel and its deferred pass
+    (fast_list_min 1), with and without bad-block passes and count rebuilds: the same
+    payloads as the oracle."""
+    print(f"analyse parity: fast coarse later sweeps {aopt}", flush=True)
+    frames = synth.clip(416, 240, 4, seed=909, vmax=12.0, nobj=(3, 5), rot_deg=2.0)
+    gpu_ctx.set_tuning(fast_list_min=1, coarse_slots=64, coarse_run=4)
+    try:
+        _check(gpu_ctx, oracle, frames, "{pel:2}", aopt, batch=True)
     finally:
         gpu_ctx.set_tuning()
 
