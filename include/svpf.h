@@ -171,10 +171,10 @@ int svpf_analyse_set_max_sweeps(svpf_ctx* ctx, int32_t max_sweeps);
  *   graphs       1 (default): the later sweeps of a level are replayed from a captured
  *                hipGraph of six sweeps (cached per launch configuration); -1: launched
  *                one by one.  The chase and svpf_analyse_pair_noalloc never use graphs;
- *   fast_list_min at a coarse level whose sweep 0 ran the straight-line tile-plan kernel, a
- *                later sweep whose list holds at least this many blocks runs that kernel
- *                too, with a deferred pass for the blocks it passes on (default 4096;
- *                -1: never). */
+ *   fast_list_min at a level whose sweep 0 ran the straight-line tile-plan kernel, a later
+ *                sweep whose list holds at least this many blocks runs that kernel too,
+ *                with a deferred pass for the blocks it passes on (default 4096; -1:
+ *                never). */
 typedef struct svpf_analyse_tuning {
     int32_t chase;
     double slot_scale;

@@ -1959,10 +1959,10 @@ __device__ __forceinline__ int order_successor(const Chain* __restrict__ chains,
 // few sweeps and each sweep waits on its slowest wave, so a second, deferred pass per
 // sweep would cost more than three waves per SIMD gain.)
 //
-// kFast (with kLean): the coarse levels' tile-plan instantiation, for the long lists of
-// the first sweeps after sweep 0; a block it cannot finish stores nothing and goes to the
-// deferred list, which deferred_kernel completes before sweep s + 1 (the host clears
-// that list's length first).  No chase.
+// kFast (with kL0 or kLean): the straight-line tile-plan instantiation, for the long
+// lists of the first sweeps after sweep 0; a block it cannot finish stores nothing and
+// goes to the deferred list, which deferred_kernel completes before sweep s + 1 (the
+// host clears that list's length first).  No chase.
 template <bool kL0, bool kFast = false, bool kLean = false>
 __global__ void __launch_bounds__(kWave, kFast ? 3 : 2) sweep_kernel(const Chain* __restrict__ chains, SweepState st,
                                                                      int s) {
@@ -2483,8 +2483,11 @@ hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, i
     return hipGetLastError();
 }
 
-hipError_t launch_sweep_fast(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s) {
-    hipLaunchKernelGGL((sweep_kernel<false, true, true>), dim3(grid), dim3(kWave), 0, s, d_chains, st, sweep);
+hipError_t launch_sweep_fast(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s, bool l0) {
+    if (l0)
+        hipLaunchKernelGGL((sweep_kernel<true, true, false>), dim3(grid), dim3(kWave), 0, s, d_chains, st, sweep);
+    else
+        hipLaunchKernelGGL((sweep_kernel<false, true, true>), dim3(grid), dim3(kWave), 0, s, d_chains, st, sweep);
     return hipGetLastError();
 }
 

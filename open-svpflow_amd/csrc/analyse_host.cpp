@@ -16,7 +16,7 @@ hipError_t launch_sweep0(const Chain* d_chains, int nchains, int max_n, const Sw
                          int level, bool fast, bool l0, bool lean, double slot_scale, int coarse_slots, int fine_run,
                          int coarse_run);
 hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s, bool l0);
-hipError_t launch_sweep_fast(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s);
+hipError_t launch_sweep_fast(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s, bool l0);
 hipError_t launch_sweep_stats(const SweepState& st, int sweeps, long long total_blocks, unsigned long long* out4,
                               hipStream_t s);
 hipError_t launch_deferred(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s);
@@ -258,12 +258,12 @@ int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total
                                         ctx->stream));
         return SVPF_OK;
     };
-    // Coarse levels whose sweep 0 ran the straight-line tile-plan kernel: while a list
-    // holds at least fast_list_min blocks, the later sweep runs that kernel too (its
-    // general instantiation takes about three times as long per block there), and
-    // deferred_kernel completes the blocks it passed on; then single launches of the
-    // general kernel up to the next graph boundary.
-    if (fast && lean && !l0 && st.chase == 0 && ctx->fast_list_min > 0) {
+    // Levels whose sweep 0 ran the straight-line tile-plan kernel: while a list holds at
+    // least fast_list_min blocks, the later sweep runs that kernel too (the general
+    // instantiation takes about three times as long per block at the coarse levels, a
+    // third longer at the finest), and deferred_kernel completes the blocks it passed on;
+    // then single launches of the general kernel up to the next graph boundary.
+    if (fast && (lean || l0) && st.chase == 0 && ctx->fast_list_min > 0) {
         for (;;) {
             SVPF_HIP(hipMemcpyAsync(h_count, st.counts + s % 3, 4, hipMemcpyDeviceToHost, ctx->stream));
             SVPF_HIP(hipStreamSynchronize(ctx->stream));
@@ -271,7 +271,7 @@ int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total
             if (*h_count < (uint32_t)ctx->fast_list_min) break;
             if (s + 1 > st.max_sweeps) return kNotConverged;
             SVPF_HIP(hipMemsetAsync(st.dcount + kDcDeferred, 0, 4, ctx->stream));
-            SVPF_HIP(launch_sweep_fast(dch, grid, st, s, ctx->stream));
+            SVPF_HIP(launch_sweep_fast(dch, grid, st, s, ctx->stream, l0));
             SVPF_HIP(launch_deferred(dch, grid, st, s, ctx->stream));
             if (scan) SVPF_HIP(launch_sweep_scan(dch, nch, st, s, ctx->stream));
             s++;

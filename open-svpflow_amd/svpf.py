@@ -194,7 +194,7 @@ class Context:
                    fine_run: int = 0, coarse_run: int = 0, graphs: int = 0, fast_list_min: int = 0):
         """svpf_analyse_set_tuning: zero keeps a default (chase 0: no chase; graphs -1: later
         sweeps launched one by one instead of replayed from a captured hipGraph;
-        fast_list_min -1: no coarse later sweep runs the tile-plan kernel)."""
+        fast_list_min -1: no later sweep runs the tile-plan kernel)."""
         t = AnalyseTuning(chase, slot_scale, coarse_slots, sweep_grid, fine_run, coarse_run, graphs, fast_list_min)
         check(lib().svpf_analyse_set_tuning(self.h, C.byref(t)))
 

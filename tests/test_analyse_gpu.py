@@ -181,11 +181,11 @@ def test_analyse_tuning_keeps_vectors(gpu_ctx, oracle, tuning):
     "{block:{w:4,h:4,overlap:0}, main:{search:{type:2,distance:6,coarse:{bad:{range:-3, sad:40}}}}}",
     "{block:{w:8,h:8,overlap:3}, main:{search:{type:2,distance:8,coarse:{distance:-6, satd:false}}}}",
 ])
-def test_analyse_fast_coarse_later_sweeps(gpu_ctx, oracle, aopt):
-    """Every coarse-level later sweep through the tile-plan kernel and its deferred pass
-    (fast_list_min 1), with and without bad-block passes and count rebuilds: the same
-    payloads as the oracle."""
-    print(f"analyse parity: fast coarse later sweeps {aopt}", flush=True)
+def test_analyse_fast_later_sweeps(gpu_ctx, oracle, aopt):
+    """Every later sweep, finest and coarse levels, through the tile-plan kernel and its
+    deferred pass (fast_list_min 1), with and without bad-block passes and count
+    rebuilds: the same payloads as the oracle."""
+    print(f"analyse parity: fast later sweeps {aopt}", flush=True)
     frames = synth.clip(416, 240, 4, seed=909, vmax=12.0, nobj=(3, 5), rot_deg=2.0)
     gpu_ctx.set_tuning(fast_list_min=1, coarse_slots=64, coarse_run=4)
     try:
