@@ -2012,6 +2012,8 @@ __global__ void __launch_bounds__(kWave, kFast ? 3 : 2) sweep_kernel(const Chain
 // and a result that differs from the provisional one is stored and propagated as any
 // change is.  So the sweep's invariant -- every block whose inputs changed is
 // evaluated again -- holds as for the others.
+// kL0: the finest level's instantiation (its general search fits three waves per SIMD).
+template <bool kL0>
 __global__ void __launch_bounds__(kWave, 2) deferred_kernel(const Chain* __restrict__ chains, SweepState st,
                                                             int s) {
     const uint32_t n = (uint32_t)uniform_int((int)__atomic_load_n(&st.dcount[kDcDeferred], __ATOMIC_RELAXED));
@@ -2019,7 +2021,7 @@ __global__ void __launch_bounds__(kWave, 2) deferred_kernel(const Chain* __restr
         const unsigned long long item = st.dl[i];
         const int ci = __builtin_amdgcn_readfirstlane((int)(item >> 32));
         const int idx = __builtin_amdgcn_readfirstlane((int)(item & 0xFFFFFFFFu));
-        sweep_item<false>(chains, ci, idx, state_here(), s);
+        sweep_item<false, kL0>(chains, ci, idx, state_here(), s);
     }
 }
 
@@ -2497,8 +2499,11 @@ hipError_t launch_sweep_stats(const SweepState& st, int sweeps, long long total_
     return hipGetLastError();
 }
 
-hipError_t launch_deferred(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s) {
-    hipLaunchKernelGGL(deferred_kernel, dim3(grid), dim3(kWave), 0, s, d_chains, st, sweep);
+hipError_t launch_deferred(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s, bool l0) {
+    if (l0)
+        hipLaunchKernelGGL(deferred_kernel<true>, dim3(grid), dim3(kWave), 0, s, d_chains, st, sweep);
+    else
+        hipLaunchKernelGGL(deferred_kernel<false>, dim3(grid), dim3(kWave), 0, s, d_chains, st, sweep);
     return hipGetLastError();
 }
 

@@ -19,7 +19,7 @@ hipError_t launch_sweep(const Chain* d_chains, int grid, const SweepState& st, i
 hipError_t launch_sweep_fast(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s, bool l0);
 hipError_t launch_sweep_stats(const SweepState& st, int sweeps, long long total_blocks, unsigned long long* out4,
                               hipStream_t s);
-hipError_t launch_deferred(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s);
+hipError_t launch_deferred(const Chain* d_chains, int grid, const SweepState& st, int sweep, hipStream_t s, bool l0);
 hipError_t launch_sweep_scan(const Chain* d_chains, int nchains, const SweepState& st, int sweep, hipStream_t s);
 hipError_t launch_validate(const Chain* d_chains, int nchains, int max_n, const SweepState& st, hipStream_t s);
 hipError_t launch_interp(const void* d_tr, int ntr, int max_n, hipStream_t s);
@@ -242,7 +242,7 @@ int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total
     const int grid = (int)std::min<size_t>(std::max<size_t>(total_blocks, 1), (size_t)ctx->sweep_grid);
     SVPF_HIP(launch_sweep0(dch, nch, max_n, st, ctx->stream, level, fast, l0, lean, ctx->slot_scale, ctx->coarse_slots,
                            ctx->fine_run, ctx->coarse_run));
-    if (fast) SVPF_HIP(launch_deferred(dch, grid, st, 0, ctx->stream));
+    if (fast) SVPF_HIP(launch_deferred(dch, grid, st, 0, ctx->stream, l0));
     if (scan) SVPF_HIP(launch_sweep_scan(dch, nch, st, 0, ctx->stream));
     SVPF_HIP(launch_validate(dch, nch, max_n, st, ctx->stream));
     // the chase claims blocks by absolute sweep number, so its sweeps are not replayable
@@ -272,7 +272,7 @@ int run_sweeps(svpf_ctx* ctx, const Chain* dch, int nch, int max_n, size_t total
             if (s + 1 > st.max_sweeps) return kNotConverged;
             SVPF_HIP(hipMemsetAsync(st.dcount + kDcDeferred, 0, 4, ctx->stream));
             SVPF_HIP(launch_sweep_fast(dch, grid, st, s, ctx->stream, l0));
-            SVPF_HIP(launch_deferred(dch, grid, st, s, ctx->stream));
+            SVPF_HIP(launch_deferred(dch, grid, st, s, ctx->stream, l0));
             if (scan) SVPF_HIP(launch_sweep_scan(dch, nch, st, s, ctx->stream));
             s++;
         }
