@@ -173,7 +173,7 @@ int svpf_analyse_set_max_sweeps(svpf_ctx* ctx, int32_t max_sweeps);
  *                one by one.  The chase and svpf_analyse_pair_noalloc never use graphs;
  *   fast_list_min at a level whose sweep 0 ran the straight-line tile-plan kernel, a later
  *                sweep whose list holds at least this many blocks runs that kernel too,
- *                with a deferred pass for the blocks it passes on (default 4096; -1:
+ *                with a deferred pass for the blocks it passes on (default 16384; -1:
  *                never). */
 typedef struct svpf_analyse_tuning {
     int32_t chase;

@@ -156,7 +156,7 @@ constexpr int kMaxSweeps = 4096;
 constexpr int kNotConverged = 1;
 
 
-constexpr int kFastListMin = 4096;  // default of svpf_analyse_tuning::fast_list_min
+constexpr int kFastListMin = 16384;  // default of svpf_analyse_tuning::fast_list_min
 constexpr int kGraphSweeps = 6;  // sweeps per captured graph: lists by s mod 2, queue lengths by s mod 3
 constexpr size_t kSweepGraphs = 16;
 

@@ -39,7 +39,7 @@ struct svpf_ctx {
     // Analyse: later sweeps replayed from captured hipGraphs (one per level launch
     // configuration, least recently used evicted beyond kSweepGraphs).
     bool sweep_graphs = true;
-    int fast_list_min = 4096;  // later sweeps with a list this long run the tile-plan kernel (-1: never)
+    int fast_list_min = 16384;  // later sweeps with a list this long run the tile-plan kernel (-1: never)
     struct SweepGraph {
         std::string key;  // the launch arguments the graph was captured with
         hipGraph_t graph = nullptr;
