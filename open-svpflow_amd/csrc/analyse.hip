@@ -1864,7 +1864,7 @@ __device__ __forceinline__ void level0_order(int pos, int gw, int gh, int& x, in
 // at the finest level consecutive meander positions keep the reference rows of a
 // block's window in the caches for the next.
 template <bool kFast, bool kL0 = false, bool kLean = false>
-__global__ void __launch_bounds__(kWave, kFast ? 3 : 2) sweep0_kernel(const Chain* __restrict__ chains,
+__global__ void __launch_bounds__(kWave, kFast ? (kLean ? 5 : 3) : 2) sweep0_kernel(const Chain* __restrict__ chains,
                                                                      SweepState st, int nslots, int nseg,
                                                                      int run, long long total) {
     for (long long u = blockIdx.x; u < total; u += gridDim.x) {
@@ -1964,7 +1964,7 @@ __device__ __forceinline__ int order_successor(const Chain* __restrict__ chains,
 // goes to the deferred list, which deferred_kernel completes before sweep s + 1 (the
 // host clears that list's length first).  No chase.
 template <bool kL0, bool kFast = false, bool kLean = false>
-__global__ void __launch_bounds__(kWave, kFast ? 3 : 2) sweep_kernel(const Chain* __restrict__ chains, SweepState st,
+__global__ void __launch_bounds__(kWave, kFast ? (kLean ? 5 : 3) : 2) sweep_kernel(const Chain* __restrict__ chains, SweepState st,
                                                                      int s) {
     const int p = s & 1;
     // this workgroup's first item, loaded with the list length rather than after it (the
