@@ -348,7 +348,7 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=64, help="frame pairs of the CPU baseline sample (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=15.0,
                     help="length of the config-5 serving run that gives the p50/p99 frame latency (0: skip)")
-    ap.add_argument("--serve-budget", type=float, default=0.4,
+    ap.add_argument("--serve-budget", type=float, default=0.5,
                     help="serving admission: fraction of the batched source pixel rate the admitted streams may deliver")
     ap.add_argument("--config4-clips", type=int, default=64, help="clips of the config-4 batched run (0: skip)")
     ap.add_argument("--config4-pairs-per-call", type=int, default=480,
