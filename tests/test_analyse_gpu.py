@@ -194,6 +194,16 @@ def test_analyse_fast_later_sweeps(gpu_ctx, oracle, aopt):
         gpu_ctx.set_tuning()
 
 
+@pytest.mark.parametrize("bad", [dict(fine_run=3), dict(coarse_run=2), dict(slot_scale=-1.0), dict(graphs=2),
+                                 dict(fast_list_min=-2), dict(sweep_grid=-1)])
+def test_analyse_tuning_rejects_out_of_range(gpu_ctx, bad):
+    """svpf_analyse_set_tuning refuses settings outside their documented ranges with an
+    error, and leaves the context usable."""
+    with pytest.raises(svpf.SvpfError, match="svpf_analyse_set_tuning"):
+        gpu_ctx.set_tuning(**bad)
+    gpu_ctx.set_tuning()
+
+
 def test_analyse_graph_replays_match_launches(gpu_ctx, oracle):
     """The later sweeps replayed from a captured hipGraph (cached per launch
     configuration, so later calls replay the graph captured by the first) give the
